@@ -1,0 +1,204 @@
+/*
+ * mdtfx.h -- C ABI of the MI355X-native MDT FX correlator (libmdtfx.so).
+ *
+ * Drop-in replacement for the four GPU calls of the MDT_OneStation_client
+ * rank-2 loop (MDT_OneStation_client.cu:1083, 1105, 1114, 1128) plus the
+ * native fused path (int8 samples -> PFB+FFT+corner turn -> triangle X-engine).
+ *
+ * Conventions (all entry points):
+ *   - every function returns an int status (MDT_SUCCESS == 0) and never aborts;
+ *     mdt_last_error_string() describes the last failure of the calling thread.
+ *     This replaces the reference's cudaErrCheck()/MPI_Abort convention
+ *     (MDT_host_functions.cu:57-71, MDT_OneStation_client.cu:1087-1092).
+ *   - device buffers are caller-owned (as in MDT_OneStation_client.cu:560-576);
+ *     kernels never allocate.  Plans/handles own only their constant tables
+ *     (window, twiddles) and are released with the matching _destroy call.
+ *   - work is stream-ordered and asynchronous on the given hipStream_t
+ *     (0 = legacy default stream); the library never synchronises the device.
+ *   - mdt_complex is layout-identical to float2 / cuComplex / cufftComplex.
+ *   - argument limits follow MDT_OneStation_client.h:54-61:
+ *     3 <= NpolsxNelements <= 4096, 32 <= Nchannels <= 16384,
+ *     4 <= Ntaps <= 128, 2 <= Nspectra <= 16384.
+ */
+#ifndef MDTFX_H
+#define MDTFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  float x, y;
+} mdt_complex;
+
+enum mdt_status {
+  MDT_SUCCESS = 0,
+  MDT_ERR_INVALID_VALUE = 1, /* shape or limit outside the contract */
+  MDT_ERR_NULL_POINTER = 2,
+  MDT_ERR_UNSUPPORTED = 3,   /* valid for the reference, not implemented here */
+  MDT_ERR_LAUNCH = 4,        /* hipGetLastError() after a launch */
+  MDT_ERR_ALLOC = 5,         /* plan/table allocation failed */
+  MDT_ERR_DEVICE = 6,        /* other HIP runtime failure */
+  MDT_ERR_MISALIGNED = 7
+};
+
+/* limits, MDT_OneStation_client.h:54-61 (inputs = Npols x Nelements) */
+#define MDT_MIN_INPUTS 3
+#define MDT_MAX_INPUTS 4096
+#define MDT_MIN_CHANNELS 32
+#define MDT_MAX_CHANNELS 16384
+#define MDT_MIN_TAPS 4
+#define MDT_MAX_TAPS 128
+#define MDT_MIN_SPECTRA 2
+#define MDT_MAX_SPECTRA 16384
+
+/* window kinds */
+#define MDT_WINDOW_BH4 0     /* sinc x min 4-term Blackman-Harris (MDT_device_functions.cu:52-88) */
+#define MDT_WINDOW_HAMMING 1 /* sinc x periodic Hamming */
+
+/* FFT direction and type, values as cufft.h (CUFFT_FORWARD, CUFFT_INVERSE, CUFFT_C2C) */
+#define MDT_FFT_FORWARD (-1)
+#define MDT_FFT_INVERSE (1)
+#define MDT_FFT_C2C 0x29
+
+/* X-engine operations, values as cublas_v2.h (CUBLAS_OP_N / _T / _C) */
+#define MDT_OP_N 0
+#define MDT_OP_T 1
+#define MDT_OP_C 2
+
+/* X-engine precisions of the native path */
+#define MDT_XPREC_FP32 0       /* v_mfma_f32_32x32x2f32 */
+#define MDT_XPREC_FP16_SPLIT 1 /* hi+lo FP16, three v_mfma_f32_32x32x16_f16 products */
+
+const char *mdt_last_error_string(void);
+int mdt_version(int *version);
+
+/* ---------------------------------------------------------------- a1 window
+ * Replaces PFB_Window_min4term_BH<<<...>>>(Nchannels, Ntaps, Window)
+ * (MDT_device_functions.cu:52-88, MDT_device_functions.h:44; called as
+ * PFB_Window_ExBlackman at MDT_OneStation_client.cu:589).  Window is a device
+ * array of Ntaps*Nchannels floats, tap-major.  Values are evaluated in FP64
+ * and rounded once to FP32. */
+int mdt_pfb_window(int Nchannels, int Ntaps, float *Window, int kind, hipStream_t stream);
+
+/* ---------------------------------------------------------------- a2 PFB FIR
+ * Replaces PpS_Batch<<<(ceil(N/1024), Nspectra, NN), 1024>>>(Nchannels, Ntaps,
+ * Window, InSignal, PolyStruct) (MDT_device_functions.cu:113-161,
+ * MDT_device_functions.h:45, launch MDT_OneStation_client.cu:1083).  The grid
+ * extents the reference passes implicitly (gridDim.y = Nspectra, gridDim.z =
+ * NpolsxNelements) are explicit arguments.  InSignal: (Nspectra+Ntaps-1) x NN
+ * x Nchannels; PolyStruct: Nspectra x NN x Nchannels.  Bit-exact: FP32 fmaf in
+ * tap order. */
+int mdt_pps_batch(int Nchannels, int Ntaps, const float *Window, const mdt_complex *InSignal,
+                  mdt_complex *PolyStruct, int NpolsxNelements, int Nspectra, hipStream_t stream);
+/* Same FIR on int8 (I, Q) input samples, same layout with 2 bytes per sample. */
+int mdt_pps_batch_i8(int Nchannels, int Ntaps, const float *Window, const int8_t *InSignal,
+                     mdt_complex *PolyStruct, int NpolsxNelements, int Nspectra,
+                     hipStream_t stream);
+
+/* ---------------------------------------------------------------- a3 FFT
+ * Replaces cufftPlan1d(&plan, Nchannels, CUFFT_C2C, Nspectra*NN),
+ * cufftSetStream, cufftExecC2C(plan, in, out, CUFFT_FORWARD)
+ * (MDT_OneStation_client.cu:622, 630, 1105).  Batched, unnormalised, natural
+ * bin order; in place or out of place.  nx must be a power of two in
+ * [8, 16384] (MDT_ERR_UNSUPPORTED otherwise). */
+typedef struct mdt_fft_plan_s *mdt_fft_plan;
+int mdt_fft_plan_1d(mdt_fft_plan *plan, int nx, int type, int batch);
+int mdt_fft_set_stream(mdt_fft_plan plan, hipStream_t stream);
+int mdt_fft_exec_c2c(mdt_fft_plan plan, mdt_complex *idata, mdt_complex *odata, int direction);
+int mdt_fft_destroy(mdt_fft_plan plan);
+
+/* ---------------------------------------------------------------- a4 corner turn
+ * Replaces ReorderFOutput<<<...>>>(NpolsxNelements, Nchannels, FOutput,
+ * XInput) (MDT_device_functions.cu:183-208, MDT_device_functions.h:46, launch
+ * MDT_OneStation_client.cu:1114).  XInput[(c*Ns + s)*NN + e] =
+ * FOutput[(s*NN + e)*N + c]; a bit-exact 8-byte permutation. */
+int mdt_reorder_foutput(int NpolsxNelements, int Nchannels, const mdt_complex *FOutput,
+                        mdt_complex *XInput, int Nspectra, hipStream_t stream);
+
+/* ---------------------------------------------------------------- a5 X-engine
+ * Replaces cublasCreate / cublasSetStream / cublasCgemmBatched
+ * (MDT_OneStation_client.cu:690, 699, 1128).  mdt_cgemm_batched takes the
+ * reference's arguments unchanged (device pointer tables, column-major):
+ *   C[i] = alpha * op(A[i]) * op(B[i]) + beta * C[i]
+ * Supported: transa = MDT_OP_N, transb = MDT_OP_C (the reference call).  When
+ * Aarray == Barray the product is Hermitian and only its lower triangle is
+ * computed (the upper one is written as the conjugate mirror). */
+typedef struct mdt_xengine_s *mdt_xengine;
+int mdt_xengine_create(mdt_xengine *handle);
+int mdt_xengine_set_stream(mdt_xengine handle, hipStream_t stream);
+int mdt_xengine_destroy(mdt_xengine handle);
+int mdt_cgemm_batched(mdt_xengine handle, int transa, int transb, int m, int n, int k,
+                      const mdt_complex *alpha, const mdt_complex *const Aarray[], int lda,
+                      const mdt_complex *const Barray[], int ldb, const mdt_complex *beta,
+                      mdt_complex *const Carray[], int ldc, int batchCount);
+/* Strided form of the same Hermitian product without pointer tables:
+ * XOutput[c*NN*NN + a + b*NN] = alpha * sum_s x[c][s][a] conj(x[c][s][b]),
+ * x[c][s][e] = XInput[(c*Nspectra + s)*NN + e]. */
+int mdt_xengine_herk(mdt_xengine handle, int NpolsxNelements, int Nspectra, float alpha,
+                     const mdt_complex *XInput, mdt_complex *XOutput, int Nchannels);
+
+/* ---------------------------------------------------------------- a6 PFB history
+ * The (Ntaps-1)-spectrum carry between dumps: copies the last Ntaps-1 spectra
+ * of the F input to its front (MDT_OneStation_client.cu:1170).  elem_bytes = 8
+ * for the float2 input of mdt_pps_batch, 2 for int8 (I, Q). */
+int mdt_history_shift(void *FInput, int NpolsxNelements, int Nchannels, int Ntaps, int Nspectra,
+                      int elem_bytes, hipStream_t stream);
+
+/* ---------------------------------------------------------------- a7 unpack
+ * Decodes the SNAP wire format: each uint32 word is a two's-complement fixed
+ * point value scaled by 2^-19, computed as in MDT_OneStation_client.cu:898-907
+ * (including its mapping of 0x80000000 to +4096.0).  n words -> n floats. */
+int mdt_unpack_u32(const uint32_t *words, float *out, size_t n, hipStream_t stream);
+
+/* ---------------------------------------------------------------- native fused path
+ * One dump: int8 (I,Q) samples [Ns+T-1][NN][N][2] -> fused PFB FIR + FFT +
+ * corner turn -> channel-major X input [N][NN][Ns] complex FP32 -> MFMA
+ * X-engine over the lower triangle -> visibilities [N][NN(NN+1)/2] complex
+ * FP32, baseline bl(a,b) = a(a+1)/2 + b for a >= b, V = (1/Ns) sum_s
+ * x_a conj(x_b) (the alpha of MDT_OneStation_client.cu:683).  Optionally adds
+ * V into an FP64 complex accumulator of the same layout (cross-dump
+ * integration).  The plan owns the window and twiddle tables only. */
+typedef struct mdt_fx_plan_s *mdt_fx_plan;
+int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, int Ntaps,
+                       int Nspectra, int window_kind, int xprec);
+int mdt_fx_set_stream(mdt_fx_plan plan, hipStream_t stream);
+int mdt_fx_set_precision(mdt_fx_plan plan, int xprec);
+/* bytes of the channel-major X input buffer the caller must provide */
+int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes);
+/* copies the plan's FP32 window [Ntaps][Nchannels] to a device buffer */
+int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
+int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
+int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_complex *Vis,
+                   double *Acc64);
+int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative,
+                mdt_complex *Vis, double *Acc64);
+int mdt_fx_plan_destroy(mdt_fx_plan plan);
+
+/* ---------------------------------------------------------------- synthetic input
+ * Deterministic counter-based generator of int8 (I, Q) samples for the
+ * operating points: x_e[t] = clip(round(sigma*(sqrt(1-rho) n_e[t] +
+ * sqrt(rho) c[t - delay_e]) + tone)), n and c unit complex Gaussians from
+ * splitmix64 + Box-Muller.  Layout [nframes][NN][N][2]; frame f holds samples
+ * t = f*N .. f*N+N-1 of every input (MDT_OneStation_client.cu:855-856). */
+typedef struct {
+  uint64_t seed;
+  float sigma;
+  float rho;        /* common-component power fraction, 0 = independent inputs */
+  int max_delay;    /* per-input integer delay of the common component, 0..max_delay */
+  int ntones;       /* up to 4 tones, added to every input */
+  float tone_freq[4]; /* cycles per sample */
+  float tone_amp[4];
+  long long t0;     /* first sample index (for streaming dumps) */
+} mdt_synth_params;
+int mdt_synth_i8(const mdt_synth_params *p, int NpolsxNelements, int Nchannels, int nframes,
+                 int8_t *out, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDTFX_H */

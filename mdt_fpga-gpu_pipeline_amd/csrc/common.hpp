@@ -1,0 +1,83 @@
+// common.hpp -- shared helpers of libmdtfx: status/error reporting, argument
+// validation against the reference limits, complex arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdint>
+
+#include "mdtfx.h"
+
+namespace mdt {
+
+// Thread-local error text, returned by mdt_last_error_string().
+void set_error_text(const char *fmt, ...);
+
+// Record an error and return its status code.
+int fail(int code, const char *fmt, ...);
+
+// Status of the last launch on this thread (hipGetLastError), as MDT_ERR_LAUNCH.
+int check_launch(const char *what);
+
+inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+inline int ilog2(long v) {
+  int l = 0;
+  while ((1L << l) < v) l++;
+  return l;
+}
+
+// Reference limits, MDT_OneStation_client.h:54-61.
+int check_inputs(int NN);
+int check_channels(int N);
+int check_taps(int T);
+int check_spectra(int Ns);
+
+// ----------------------------------------------------------- complex helpers
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// Exact-argument sin(pi x), cos(pi x) in FP64: exact zeros at the integers
+// (sin) and half-integers (cos), so windowed-sinc zeros are exact.
+__device__ __forceinline__ double dsinpi(double x) {
+  double n = rint(x);
+  double r = x - n;
+  if (r == 0.0) return 0.0;
+  double s = (fabs(r) <= 0.25) ? sin(M_PI * r) : copysign(cos(M_PI * (0.5 - fabs(r))), r);
+  return (fmod(fabs(n), 2.0) == 1.0) ? -s : s;
+}
+__device__ __forceinline__ double dcospi(double x) {
+  double n = rint(x);
+  double r = fabs(x - n);
+  double c = (r <= 0.25) ? cos(M_PI * r) : sin(M_PI * (0.5 - r));
+  return (fmod(fabs(n), 2.0) == 1.0) ? -c : c;
+}
+
+}  // namespace mdt
+
+#define MDT_HIP_TRY(call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ::mdt::fail(MDT_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define MDT_REQUIRE_PTR(p)                                                  \
+  do {                                                                      \
+    if ((p) == nullptr) return ::mdt::fail(MDT_ERR_NULL_POINTER, "%s is NULL", #p); \
+  } while (0)
+
+#define MDT_TRY(expr)            \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ != MDT_SUCCESS) return s_; \
+  } while (0)

@@ -1,0 +1,303 @@
+// core.hip -- error reporting, argument limits, PFB window (a1), corner turn
+// (a4), PFB history carry (a6), SNAP word unpack (a7), synthetic int8 input.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "fft_core.hpp"
+
+namespace mdt {
+
+static thread_local char g_err[512] = "no error";
+
+void set_error_text(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MDT_ERR_LAUNCH, "%s launch failed: %s", what, hipGetErrorString(e));
+  return MDT_SUCCESS;
+}
+
+// MDT_OneStation_client.h:54-61 (inputs = Npols * Nelements, Npols in {1, 2})
+int check_inputs(int NN) {
+  if (NN < MDT_MIN_INPUTS || NN > MDT_MAX_INPUTS)
+    return fail(MDT_ERR_INVALID_VALUE, "NpolsxNelements=%d outside [%d, %d]", NN, MDT_MIN_INPUTS,
+                MDT_MAX_INPUTS);
+  return MDT_SUCCESS;
+}
+int check_channels(int N) {
+  if (N < MDT_MIN_CHANNELS || N > MDT_MAX_CHANNELS)
+    return fail(MDT_ERR_INVALID_VALUE, "Nchannels=%d outside [%d, %d]", N, MDT_MIN_CHANNELS,
+                MDT_MAX_CHANNELS);
+  return MDT_SUCCESS;
+}
+int check_taps(int T) {
+  if (T < MDT_MIN_TAPS || T > MDT_MAX_TAPS)
+    return fail(MDT_ERR_INVALID_VALUE, "Ntaps=%d outside [%d, %d]", T, MDT_MIN_TAPS, MDT_MAX_TAPS);
+  return MDT_SUCCESS;
+}
+int check_spectra(int Ns) {
+  if (Ns < MDT_MIN_SPECTRA || Ns > MDT_MAX_SPECTRA)
+    return fail(MDT_ERR_INVALID_VALUE, "Nspectra=%d outside [%d, %d]", Ns, MDT_MIN_SPECTRA,
+                MDT_MAX_SPECTRA);
+  return MDT_SUCCESS;
+}
+
+// ------------------------------------------------------------------ twiddles
+__global__ void twiddle_kernel(int N, float2 *tw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < N) {
+    double a = 2.0 * (double)k / (double)N;
+    tw[k] = make_float2((float)dcospi(a), (float)(-dsinpi(a)));
+  }
+}
+
+int make_twiddles(int N, float2 **dev_table) {
+  float2 *p = nullptr;
+  if (hipMalloc(&p, sizeof(float2) * N) != hipSuccess)
+    return fail(MDT_ERR_ALLOC, "twiddle table of %d entries", N);
+  hipLaunchKernelGGL(twiddle_kernel, dim3((N + 255) / 256), dim3(256), 0, 0, N, p);
+  int st = check_launch("twiddle_kernel");
+  if (st != MDT_SUCCESS) {
+    (void)hipFree(p);
+    return st;
+  }
+  if (hipStreamSynchronize(0) != hipSuccess) {
+    (void)hipFree(p);
+    return fail(MDT_ERR_DEVICE, "twiddle table build");
+  }
+  *dev_table = p;
+  return MDT_SUCCESS;
+}
+
+// ------------------------------------------------------------------ a1 window
+// MDT_device_functions.cu:52-88, evaluated in FP64 and rounded once to FP32:
+// W[i] = sinc((i - TN/2)/N) * w(2i/TN), sinc(0) = 1.
+__global__ void window_kernel(int N, int T, int kind, float *W) {
+  long TN = (long)T * N;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= TN) return;
+  double x = ((double)i - 0.5 * (double)TN) / (double)N;
+  double sinc = (x == 0.0) ? 1.0 : dsinpi(x) / (M_PI * x);
+  double u = 2.0 * (double)i / (double)TN;
+  double w;
+  if (kind == MDT_WINDOW_HAMMING)
+    w = 0.54 - 0.46 * dcospi(u);
+  else
+    w = 0.35875 - 0.48829 * dcospi(u) + 0.14128 * dcospi(2.0 * u) - 0.01168 * dcospi(3.0 * u);
+  W[i] = (float)(sinc * w);
+}
+
+int launch_window(int N, int T, float *W, int kind, hipStream_t stream) {
+  long TN = (long)T * N;
+  hipLaunchKernelGGL(window_kernel, dim3((unsigned)((TN + 255) / 256)), dim3(256), 0, stream, N, T,
+                     kind, W);
+  return check_launch("window_kernel");
+}
+
+// ------------------------------------------------------------------ a4 corner turn
+// XIn[(c*Ns + s)*NN + e] = FOut[(s*NN + e)*N + c] (MDT_device_functions.cu:197, 207).
+// 64 channels x 32 inputs per tile through LDS; elements moved as raw 64-bit
+// words so NaN payloads and -0 survive bit for bit.
+constexpr int RT_C = 64, RT_E = 32;
+__global__ __launch_bounds__(256) void reorder_kernel(int NN, int N, int Ns,
+                                                      const unsigned long long *__restrict__ F,
+                                                      unsigned long long *__restrict__ X) {
+  __shared__ unsigned long long tile[RT_E][RT_C + 1];
+  const int c0 = blockIdx.x * RT_C, e0 = blockIdx.y * RT_E;
+  const long s = blockIdx.z;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+#pragma unroll
+  for (int k = 0; k < RT_E / 4; k++) {
+    int e = e0 + ty + 4 * k, c = c0 + tx;
+    if (e < NN && c < N) tile[ty + 4 * k][tx] = F[(s * NN + e) * (long)N + c];
+  }
+  __syncthreads();
+  // write: 32 consecutive inputs per channel row
+  const int wx = threadIdx.x & 31, wy = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+  for (int k = 0; k < RT_C / 8; k++) {
+    int c = c0 + wy + 8 * k, e = e0 + wx;
+    if (e < NN && c < N) X[((long)c * Ns + s) * NN + e] = tile[wx][wy + 8 * k];
+  }
+}
+
+// ------------------------------------------------------------------ a7 unpack
+// MDT_OneStation_client.cu:898-907: w >= 2^31 -> -2^-19 * (int)(2^32 - w),
+// else 2^-19 * (int)w, product in double, stored as float.  (int)(2^31) wraps
+// to INT_MIN, so 0x80000000 -> +4096.0.  Here: the wrapped int32 value v,
+// negated for w >= 2^31, times 2^-19 in double, rounded to float.
+__device__ __forceinline__ float unpack_word(uint32_t w) {
+  if (w >= 2147483648u) {
+    int v = (int)(uint32_t)(0u - w);  // 2^32 - w, modulo 2^32
+    return (float)(-1.9073486328125e-06 * (double)v);
+  }
+  return (float)(1.9073486328125e-06 * (double)(int)w);
+}
+
+__global__ void unpack_kernel(const uint32_t *__restrict__ w, float *__restrict__ out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = unpack_word(w[i]);
+}
+
+// ------------------------------------------------------------------ synthetic input
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// unit complex Gaussian (E|g|^2 = 1) from a counter
+__device__ __forceinline__ double2 cgauss(uint64_t seed, uint64_t stream, uint64_t ctr) {
+  uint64_t h = splitmix64(seed ^ splitmix64(stream * 0xD1B54A32D192ED03ull + ctr));
+  uint64_t h2 = splitmix64(h);
+  double u1 = ((h >> 11) + 1) * (1.0 / 9007199254740992.0);  // (0, 1]
+  double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);
+  double r = sqrt(-log(u1));  // variance 1/2 per component
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  return make_double2(r * c, r * s);
+}
+
+__device__ __forceinline__ int8_t clip8(double v) {
+  double r = rint(v);
+  r = r < -128.0 ? -128.0 : (r > 127.0 ? 127.0 : r);
+  return (int8_t)(int)r;
+}
+
+__global__ void synth_kernel(mdt_synth_params p, int NN, int N, long total, int8_t *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int n = (int)(i % N);
+  long fe = i / N;
+  int e = (int)(fe % NN);
+  long f = fe / NN;
+  long long t = p.t0 + f * (long long)N + n;
+  double2 g = cgauss(p.seed, 1 + (uint64_t)e, (uint64_t)t);
+  double re = 0.0, im = 0.0;
+  double a = sqrt(1.0 - (double)p.rho), b = sqrt((double)p.rho);
+  re = a * g.x;
+  im = a * g.y;
+  if (p.rho > 0.0f) {
+    int d = 0;
+    if (p.max_delay > 0) d = (int)(splitmix64(p.seed ^ (0xA5A5ull + (uint64_t)e)) % (uint64_t)(p.max_delay + 1));
+    double2 c = cgauss(p.seed, 0, (uint64_t)(t - d));
+    re += b * c.x;
+    im += b * c.y;
+  }
+  re *= p.sigma;
+  im *= p.sigma;
+  for (int k = 0; k < p.ntones && k < 4; k++) {
+    double s, c;
+    double ph = 2.0 * fmod((double)p.tone_freq[k] * (double)t, 1.0);
+    sincospi(ph, &s, &c);
+    re += p.tone_amp[k] * c;
+    im += p.tone_amp[k] * s;
+  }
+  out[2 * i] = clip8(re);
+  out[2 * i + 1] = clip8(im);
+}
+
+}  // namespace mdt
+
+using namespace mdt;
+
+extern "C" {
+
+const char *mdt_last_error_string(void) { return g_err; }
+
+int mdt_version(int *version) {
+  MDT_REQUIRE_PTR(version);
+  *version = 100;  // 1.0.0
+  return MDT_SUCCESS;
+}
+
+int mdt_pfb_window(int Nchannels, int Ntaps, float *Window, int kind, hipStream_t stream) {
+  MDT_TRY(check_channels(Nchannels));
+  MDT_TRY(check_taps(Ntaps));
+  MDT_REQUIRE_PTR(Window);
+  if (kind != MDT_WINDOW_BH4 && kind != MDT_WINDOW_HAMMING)
+    return fail(MDT_ERR_INVALID_VALUE, "unknown window kind %d", kind);
+  return launch_window(Nchannels, Ntaps, Window, kind, stream);
+}
+
+int mdt_reorder_foutput(int NpolsxNelements, int Nchannels, const mdt_complex *FOutput,
+                        mdt_complex *XInput, int Nspectra, hipStream_t stream) {
+  MDT_TRY(check_inputs(NpolsxNelements));
+  MDT_TRY(check_channels(Nchannels));
+  MDT_TRY(check_spectra(Nspectra));
+  MDT_REQUIRE_PTR(FOutput);
+  MDT_REQUIRE_PTR(XInput);
+  if (((uintptr_t)FOutput | (uintptr_t)XInput) & 7)
+    return fail(MDT_ERR_MISALIGNED, "complex buffers must be 8-byte aligned");
+  dim3 grid((Nchannels + RT_C - 1) / RT_C, (NpolsxNelements + RT_E - 1) / RT_E, Nspectra);
+  hipLaunchKernelGGL(reorder_kernel, grid, dim3(256), 0, stream, NpolsxNelements, Nchannels, Nspectra,
+                     (const unsigned long long *)FOutput, (unsigned long long *)XInput);
+  return check_launch("reorder_kernel");
+}
+
+int mdt_history_shift(void *FInput, int NpolsxNelements, int Nchannels, int Ntaps, int Nspectra,
+                      int elem_bytes, hipStream_t stream) {
+  MDT_TRY(check_inputs(NpolsxNelements));
+  MDT_TRY(check_channels(Nchannels));
+  MDT_TRY(check_taps(Ntaps));
+  MDT_TRY(check_spectra(Nspectra));
+  MDT_REQUIRE_PTR(FInput);
+  if (elem_bytes != 8 && elem_bytes != 2)
+    return fail(MDT_ERR_INVALID_VALUE, "elem_bytes must be 8 (float2) or 2 (int8 I/Q)");
+  // MDT_OneStation_client.cu:1170: copy spectra [Ns, Ns+T-1) to [0, T-1).
+  // The ranges overlap when Ns < T-1, so copy spectrum by spectrum in order.
+  size_t spec = (size_t)NpolsxNelements * Nchannels * elem_bytes;
+  char *base = (char *)FInput;
+  if (Nspectra >= Ntaps - 1) {
+    MDT_HIP_TRY(hipMemcpyAsync(base, base + (size_t)Nspectra * spec, (size_t)(Ntaps - 1) * spec,
+                               hipMemcpyDeviceToDevice, stream));
+  } else {
+    for (int k = 0; k < Ntaps - 1; k++)
+      MDT_HIP_TRY(hipMemcpyAsync(base + (size_t)k * spec, base + (size_t)(Nspectra + k) * spec, spec,
+                                 hipMemcpyDeviceToDevice, stream));
+  }
+  return MDT_SUCCESS;
+}
+
+int mdt_unpack_u32(const uint32_t *words, float *out, size_t n, hipStream_t stream) {
+  MDT_REQUIRE_PTR(words);
+  MDT_REQUIRE_PTR(out);
+  if (n == 0) return MDT_SUCCESS;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, words, out, n);
+  return check_launch("unpack_kernel");
+}
+
+int mdt_synth_i8(const mdt_synth_params *p, int NpolsxNelements, int Nchannels, int nframes,
+                 int8_t *out, hipStream_t stream) {
+  MDT_REQUIRE_PTR(p);
+  MDT_REQUIRE_PTR(out);
+  if (NpolsxNelements < 1 || Nchannels < 1 || nframes < 1)
+    return fail(MDT_ERR_INVALID_VALUE, "synth shape must be positive");
+  if (p->ntones < 0 || p->ntones > 4 || p->rho < 0.0f || p->rho > 1.0f || p->max_delay < 0)
+    return fail(MDT_ERR_INVALID_VALUE, "synth parameters out of range");
+  long total = (long)nframes * NpolsxNelements * Nchannels;
+  hipLaunchKernelGGL(synth_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, *p,
+                     NpolsxNelements, Nchannels, total, out);
+  return check_launch("synth_kernel");
+}
+
+}  // extern "C"

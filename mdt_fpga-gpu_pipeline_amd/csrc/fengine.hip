@@ -1,0 +1,200 @@
+// fengine.hip -- the native fused F-engine: int8 (I, Q) samples -> polyphase
+// FIR (PpS_Batch, MDT_device_functions.cu:113-161) -> C2C FFT
+// (cufftExecC2C, MDT_OneStation_client.cu:1105) -> corner turn
+// (ReorderFOutput, MDT_device_functions.cu:183-208) in one kernel, one HBM
+// read of the samples and one HBM write of the channelised output.
+//
+// Workgroup = one input e (or EB inputs for short transforms) x SB consecutive
+// spectra.  Thread t owns FFT points n = t + m*TPB.  For every frame f of the
+// SB + T - 1 needed, the thread loads its P int8 samples once and
+// fma-accumulates them into the SB outputs that use the frame, in tap order
+// (bit-identical to mdt_pps_batch_i8).  Each of the SB spectra is then
+// transformed with the shared Stockham core (bit-identical to
+// mdt_fft_exec_c2c), and the SB x N results are corner-turned through LDS so
+// that each store instruction writes 64-byte runs of SB consecutive spectra of
+// one (channel, input) row of the channel-major output [N][NN][Ns].
+#include "fengine.hpp"
+#include "fft_core.hpp"
+
+namespace mdt {
+
+template <int N>
+struct FCfg {
+  static constexpr int P = fft_points(N);
+  static constexpr int TPB = N / P;
+  static constexpr int EB = TPB >= 64 ? 1 : 64 / TPB;  // inputs per block
+  static constexpr int THREADS = TPB * EB;
+  static constexpr int SB = N <= 4096 ? 8 : (N == 8192 ? 2 : 1);  // spectra per block
+  static constexpr int FFT_LDS = lds_entries<N>();
+  static constexpr int ROW = SB + 1;  // padded staging row (float2)
+  static constexpr int STAGE_LDS = TPB * ROW;
+  static constexpr int SLICE = FFT_LDS > STAGE_LDS ? FFT_LDS : STAGE_LDS;
+};
+
+template <int N, int T>
+__global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
+  using C = FCfg<N>;
+  constexpr int P = C::P, TPB = C::TPB, SB = C::SB;
+  __shared__ float2 lds_all[C::EB * C::SLICE];
+
+  // XCD-aware: consecutive spectrum blocks of one input on one XCD (shared
+  // FIR halo frames, adjacent output segments merge in the same L2).
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+  const int nsb = (a.Ns + SB - 1) / SB;
+  const int sb = (int)(logical % nsb);
+  const int eg = (int)(logical / nsb);
+  const int t = threadIdx.x % TPB;
+  const int ey = threadIdx.x / TPB;
+  const int e = eg * C::EB + ey;
+  const bool valid_e = e < a.NN;
+  const int s0 = sb * SB;
+  const int scount = min(SB, a.Ns - s0);
+  float2 *lds = lds_all + ey * C::SLICE;
+  const int Tn = (T > 0) ? T : a.T;
+
+  float2 acc[SB][P];
+#pragma unroll
+  for (int sl = 0; sl < SB; sl++)
+#pragma unroll
+    for (int m = 0; m < P; m++) acc[sl][m] = make_float2(0.f, 0.f);
+
+  const long spec = (long)a.NN * N;
+  const char2 *src = a.in + (long)s0 * spec + (long)(valid_e ? e : 0) * N + t;
+  const float *Wt = a.W + t;
+
+  if constexpr (T > 0) {
+    // frame f feeds outputs sl = f - i for taps i = 0..T-1; f ascending means
+    // each output accumulates its taps in order i = 0..T-1.
+#pragma unroll
+    for (int f = 0; f < SB + T - 1; f++) {
+      if (f < scount + T - 1 && valid_e) {
+        float2 x[P];
+#pragma unroll
+        for (int m = 0; m < P; m++) {
+          char2 v = src[(long)f * spec + m * TPB];
+          x[m] = make_float2((float)v.x, (float)v.y);
+        }
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) {
+          const int i = f - sl;
+          if (i >= 0 && i < T && sl < scount) {
+#pragma unroll
+            for (int m = 0; m < P; m++) {
+              const float wv = Wt[(long)i * N + m * TPB];
+              acc[sl][m].x = __builtin_fmaf(wv, x[m].x, acc[sl][m].x);
+              acc[sl][m].y = __builtin_fmaf(wv, x[m].y, acc[sl][m].y);
+            }
+          }
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int sl = 0; sl < SB; sl++) {
+      if (sl < scount && valid_e) {
+        for (int i = 0; i < Tn; i++) {
+#pragma unroll
+          for (int m = 0; m < P; m++) {
+            char2 v = src[(long)(sl + i) * spec + m * TPB];
+            const float wv = Wt[(long)i * N + m * TPB];
+            acc[sl][m].x = __builtin_fmaf(wv, (float)v.x, acc[sl][m].x);
+            acc[sl][m].y = __builtin_fmaf(wv, (float)v.y, acc[sl][m].y);
+          }
+        }
+      }
+    }
+  }
+
+  // FFT of each spectrum (block-uniform trip count: scount is uniform)
+#pragma unroll
+  for (int sl = 0; sl < SB; sl++)
+    if (sl < scount) fft_block<N, P, false>(acc[sl], t, lds, a.tw);
+
+  // optional per-input complex gain and delay (calibration), per channel k:
+  // X'[k] = g_e exp(-2 pi i kk tau_e / N) X[k], kk the signed bin
+  if (a.gain != nullptr && valid_e) {
+    const float2 g = a.gain[e];
+    const float tau = a.delay ? a.delay[e] : 0.0f;
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+      const int k = t + m * TPB;
+      const int kk = (k < N / 2) ? k : k - N;
+      float sn, cs;
+      sincospif(-2.0f * (float)kk * tau / (float)N, &sn, &cs);
+      const float2 ph = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
+#pragma unroll
+      for (int sl = 0; sl < SB; sl++) acc[sl][m] = cmul(acc[sl][m], ph);
+    }
+  }
+
+  // corner turn: out[(k*NN + e)*Ns + s0 + sl], staged per m through LDS
+  if constexpr (SB == 1) {
+    if (valid_e) {
+#pragma unroll
+      for (int m = 0; m < P; m++) a.out[((long)(t + m * TPB) * a.NN + e) * a.Ns + s0] = acc[0][m];
+    }
+  } else {
+    constexpr int CH = SB / 2;  // 16-byte chunks per (k, e) row segment
+    const bool vec = ((a.Ns & 1) == 0);
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+      __syncthreads();
+#pragma unroll
+      for (int sl = 0; sl < SB; sl++) lds[t * C::ROW + sl] = acc[sl][m];
+      __syncthreads();
+      if (valid_e) {
+#pragma unroll
+        for (int v = 0; v < CH; v++) {
+          const int q = t + v * TPB;
+          const int tt = q / CH, pr = q % CH;
+          const int k = tt + m * TPB;
+          const int sl = 2 * pr;
+          float2 *dst = a.out + ((long)k * a.NN + e) * a.Ns + s0 + sl;
+          const float2 v0 = lds[tt * C::ROW + sl], v1 = lds[tt * C::ROW + sl + 1];
+          if (vec && sl + 1 < scount) {
+            *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+          } else {
+            if (sl < scount) dst[0] = v0;
+            if (sl + 1 < scount) dst[1] = v1;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int N>
+static int launch_fengine_n(const FArgs &a, hipStream_t st) {
+  using C = FCfg<N>;
+  const long nsb = (a.Ns + C::SB - 1) / C::SB;
+  const long neg = (a.NN + C::EB - 1) / C::EB;
+  const long total = nsb * neg;
+  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "F-engine grid too large");
+  dim3 grid((unsigned)total), block(C::THREADS);
+  switch (a.T) {
+    case 4: hipLaunchKernelGGL((fengine_kernel<N, 4>), grid, block, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((fengine_kernel<N, 8>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((fengine_kernel<N, 0>), grid, block, 0, st, a); break;
+  }
+  return check_launch("fengine_kernel");
+}
+
+int launch_fengine(const FArgs &a, int N, hipStream_t st) {
+  switch (N) {
+    case 32: return launch_fengine_n<32>(a, st);
+    case 64: return launch_fengine_n<64>(a, st);
+    case 128: return launch_fengine_n<128>(a, st);
+    case 256: return launch_fengine_n<256>(a, st);
+    case 512: return launch_fengine_n<512>(a, st);
+    case 1024: return launch_fengine_n<1024>(a, st);
+    case 2048: return launch_fengine_n<2048>(a, st);
+    case 4096: return launch_fengine_n<4096>(a, st);
+    case 8192: return launch_fengine_n<8192>(a, st);
+    case 16384: return launch_fengine_n<16384>(a, st);
+    default: return fail(MDT_ERR_UNSUPPORTED, "Nchannels=%d: the fused F-engine needs a power of two", N);
+  }
+}
+
+}  // namespace mdt

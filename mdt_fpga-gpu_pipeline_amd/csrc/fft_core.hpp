@@ -1,0 +1,171 @@
+// fft_core.hpp -- batched C2C FFT building block shared by the standalone
+// FFT (drop-in for cufftExecC2C, MDT_OneStation_client.cu:1105) and the fused
+// F-engine.
+//
+// Algorithm: Stockham autosort, radix-P passes plus one remainder pass.  One
+// transform of N points is held by TPB = N/P threads, P points each; thread t
+// owns elements t + m*TPB (m = 0..P-1) on entry AND owns X[t + m*TPB] on exit,
+// so the first pass reads straight from registers (or global memory, fully
+// coalesced) and the last pass writes straight to registers.  Passes in between
+// exchange through LDS.  For the pass of radix R at stage NS (product of the
+// earlier radices), butterfly j reads v[r] = x[j + r*N/R], multiplies by
+// W_{NS*R}^{(j mod NS)*r}, does an R-point DFT and stores v[r] at
+// (j/NS)*NS*R + (j mod NS) + r*NS.  Twiddles W_N^k come from a table evaluated
+// in FP64 and rounded once to FP32.
+#pragma once
+
+#include "common.hpp"
+
+namespace mdt {
+
+constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+constexpr int full_passes(int n, int p) {
+  int l = 0;
+  while (n > 1 && n % p == 0) {
+    n /= p;
+    l++;
+  }
+  return l;
+}
+
+// points per thread for an N-point transform
+constexpr int fft_points(int N) { return N <= 4096 ? 8 : 16; }
+
+// W_16^k constants, k = 0..7: cos(2 pi k / 16), sin(2 pi k / 16)
+__device__ constexpr float kCos16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f,
+                                        0.38268343236508978f, 0.0f, -0.38268343236508978f,
+                                        -0.70710678118654752f, -0.92387953251128674f};
+__device__ constexpr float kSin16[8] = {0.0f, 0.38268343236508978f, 0.70710678118654752f,
+                                        0.92387953251128674f, 1.0f, 0.92387953251128674f,
+                                        0.70710678118654752f, 0.38268343236508978f};
+
+// z * W_R^k, W_R = exp(-2 pi i / R) (forward) or exp(+2 pi i / R) (inverse);
+// k < R/2, R <= 16.  The quarter-turn case is an exact swap.
+template <int R, int K, bool INV>
+__device__ __forceinline__ float2 rot(float2 z) {
+  constexpr int idx = K * (16 / R);  // angle 2 pi idx / 16
+  if constexpr (idx == 0) {
+    return z;
+  } else if constexpr (idx == 4) {
+    return INV ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+  } else {
+    const float c = kCos16[idx], s = kSin16[idx];
+    if constexpr (INV)
+      return make_float2(z.x * c - z.y * s, z.y * c + z.x * s);
+    else
+      return make_float2(z.x * c + z.y * s, z.y * c - z.x * s);
+  }
+}
+
+// In-register R-point DFT (natural order in and out), radix-2 split.
+template <int R, bool INV>
+struct Dft {
+  template <int K>
+  __device__ __forceinline__ static void combine(float2 *v, const float2 *e, const float2 *o) {
+    if constexpr (K < R / 2) {
+      float2 t = rot<R, K, INV>(o[K]);
+      v[K] = cadd(e[K], t);
+      v[K + R / 2] = csub(e[K], t);
+      combine<K + 1>(v, e, o);
+    }
+  }
+  __device__ __forceinline__ static void run(float2 *v) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int k = 0; k < R / 2; k++) {
+      e[k] = v[2 * k];
+      o[k] = v[2 * k + 1];
+    }
+    Dft<R / 2, INV>::run(e);
+    Dft<R / 2, INV>::run(o);
+    combine<0>(v, e, o);
+  }
+};
+template <bool INV>
+struct Dft<2, INV> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <bool INV>
+struct Dft<1, INV> {
+  __device__ __forceinline__ static void run(float2 *) {}
+};
+
+// LDS index padding: one float2 of padding every 16 entries breaks the
+// power-of-two strides of the Stockham scatter across banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+template <int N>
+constexpr int lds_entries() {
+  return N + N / 16;
+}
+
+template <int N, int P, int R, int NS, bool LAST, bool INV>
+__device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds,
+                                         const float2 *__restrict__ tw) {
+  constexpr int TPB = N / P;
+  constexpr int Q = P / R;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int j = t + q * TPB;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = x[q + r * Q];
+    const int jm = j & (NS - 1);
+    if constexpr (NS > 1) {
+      constexpr int STEP = N / (NS * R);
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        float2 w = tw[jm * r * STEP];
+        if constexpr (INV) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    Dft<R, INV>::run(v);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; r++) x[q + r * Q] = v[r];
+    } else {
+      const int base = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; r++) lds[lpad(base + r * NS)] = v[r];
+    }
+  }
+}
+
+template <int N, int P, bool INV, int PASS>
+__device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds,
+                                          const float2 *__restrict__ tw) {
+  constexpr int L = full_passes(N, P);
+  constexpr int RL = N / ipow_c(P, L);
+  constexpr int NPASS = L + (RL > 1 ? 1 : 0);
+  constexpr int R = (PASS < L) ? P : RL;
+  constexpr int NS = ipow_c(P, PASS);
+  constexpr int TPB = N / P;
+  if constexpr (PASS > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < P; m++) x[m] = lds[lpad(t + m * TPB)];
+    __syncthreads();
+  }
+  fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x, t, lds, tw);
+  if constexpr (PASS + 1 < NPASS) fft_stage<N, P, INV, PASS + 1>(x, t, lds, tw);
+}
+
+// Transform one N-point vector held as x[m] = element t + m*(N/P).  All
+// threads of the block must call this the same number of times (it contains
+// __syncthreads when N > P).  lds: lds_entries<N>() float2 private to this
+// transform.
+template <int N, int P, bool INV>
+__device__ __forceinline__ void fft_block(float2 (&x)[P], int t, float2 *lds,
+                                          const float2 *__restrict__ tw) {
+  static_assert(N % P == 0 && N >= P, "N must be a multiple of P");
+  fft_stage<N, P, INV, 0>(x, t, lds, tw);
+}
+
+// Twiddle table W_N^k = exp(-2 pi i k / N), k < N, FP64 evaluated, FP32 stored.
+int make_twiddles(int N, float2 **dev_table);
+
+}  // namespace mdt

@@ -1,0 +1,178 @@
+// fxplan.hip -- the native one-dump plan: window + twiddle tables and the
+// F-engine -> X-engine sequence of the MDT_OneStation_client rank-2 loop
+// (MDT_OneStation_client.cu:1083-1128) as two kernels on one stream.
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "fengine.hpp"
+#include "fft_core.hpp"
+#include "xengine.hpp"
+
+namespace mdt {
+int launch_window(int N, int T, float *W, int kind, hipStream_t stream);
+}
+
+struct mdt_fx_plan_s {
+  int NN = 0, N = 0, T = 0, Ns = 0, kind = 0, xprec = 0;
+  hipStream_t stream = 0;
+  float *W = nullptr;
+  float2 *tw = nullptr;
+  const float2 *gain = nullptr;
+  const float *delay = nullptr;
+  float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15
+};
+
+using namespace mdt;
+
+// Rigorous bound of the channelised samples for int8 input: |X[k]| <=
+// sum_n |y[n]| <= |x|max * sum_{i,n} |W[i][n]|, |x|max = |(-128, -128)|.  The
+// FP16 split pre-scale is the power of two that maps this bound to <= 2^15,
+// below the FP16 maximum 65504; with gains applied the bound grows by max|g|.
+static int choose_f16_scale(mdt_fx_plan p, float gmax = 1.0f) {
+  std::vector<float> w((size_t)p->T * p->N);
+  if (hipMemcpy(w.data(), p->W, sizeof(float) * w.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(MDT_ERR_DEVICE, "window read-back");
+  double sum = 0.0;
+  for (float v : w) sum += std::fabs((double)v);
+  const double bound = 181.02 * sum * (double)gmax;
+  const int e = (int)std::floor(std::log2(32768.0 / bound));
+  p->f16_scale = std::ldexp(1.0f, e);
+  return MDT_SUCCESS;
+}
+
+static int check_xprec(int xprec) {
+  if (xprec != MDT_XPREC_FP32 && xprec != MDT_XPREC_FP16_SPLIT)
+    return fail(MDT_ERR_INVALID_VALUE, "unknown X-engine precision %d", xprec);
+  return MDT_SUCCESS;
+}
+
+extern "C" {
+
+int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, int Ntaps,
+                       int Nspectra, int window_kind, int xprec) {
+  MDT_REQUIRE_PTR(plan);
+  *plan = nullptr;
+  MDT_TRY(check_inputs(NpolsxNelements));
+  MDT_TRY(check_channels(Nchannels));
+  MDT_TRY(check_taps(Ntaps));
+  MDT_TRY(check_spectra(Nspectra));
+  MDT_TRY(check_xprec(xprec));
+  if (!is_pow2(Nchannels))
+    return fail(MDT_ERR_UNSUPPORTED, "Nchannels=%d: the FFT needs a power of two", Nchannels);
+  if (window_kind != MDT_WINDOW_BH4 && window_kind != MDT_WINDOW_HAMMING)
+    return fail(MDT_ERR_INVALID_VALUE, "unknown window kind %d", window_kind);
+  mdt_fx_plan p = new (std::nothrow) mdt_fx_plan_s;
+  if (!p) return fail(MDT_ERR_ALLOC, "plan");
+  p->NN = NpolsxNelements;
+  p->N = Nchannels;
+  p->T = Ntaps;
+  p->Ns = Nspectra;
+  p->kind = window_kind;
+  p->xprec = xprec;
+  if (hipMalloc(&p->W, sizeof(float) * (size_t)Ntaps * Nchannels) != hipSuccess) {
+    delete p;
+    return fail(MDT_ERR_ALLOC, "window table");
+  }
+  int st = launch_window(Nchannels, Ntaps, p->W, window_kind, 0);
+  if (st == MDT_SUCCESS) st = make_twiddles(Nchannels, &p->tw);
+  if (st == MDT_SUCCESS && hipStreamSynchronize(0) != hipSuccess)
+    st = fail(MDT_ERR_DEVICE, "plan table build");
+  if (st == MDT_SUCCESS) st = choose_f16_scale(p);
+  if (st != MDT_SUCCESS) {
+    (void)hipFree(p->W);
+    if (p->tw) (void)hipFree(p->tw);
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_set_stream(mdt_fx_plan plan, hipStream_t stream) {
+  MDT_REQUIRE_PTR(plan);
+  plan->stream = stream;
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_set_precision(mdt_fx_plan plan, int xprec) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_TRY(check_xprec(xprec));
+  plan->xprec = xprec;
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(bytes);
+  *bytes = sizeof(float2) * (size_t)plan->N * plan->NN * plan->Ns;
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_get_window(mdt_fx_plan plan, float *Window) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(Window);
+  MDT_HIP_TRY(hipMemcpyAsync(Window, plan->W, sizeof(float) * (size_t)plan->T * plan->N,
+                             hipMemcpyDeviceToDevice, plan->stream));
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(InSignal);
+  MDT_REQUIRE_PTR(XInputNative);
+  if (((uintptr_t)InSignal & 1) || ((uintptr_t)XInputNative & 15))
+    return fail(MDT_ERR_MISALIGNED, "input must be 2-byte, X input 16-byte aligned");
+  FArgs a{};
+  a.in = (const char2 *)InSignal;
+  a.W = plan->W;
+  a.tw = plan->tw;
+  a.out = (float2 *)XInputNative;
+  a.NN = plan->NN;
+  a.Ns = plan->Ns;
+  a.T = plan->T;
+  a.gain = plan->gain;
+  a.delay = plan->delay;
+  return launch_fengine(a, plan->N, plan->stream);
+}
+
+int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_complex *Vis,
+                   double *Acc64) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(XInputNative);
+  MDT_REQUIRE_PTR(Vis);
+  XArgs g{};
+  const int NN = plan->NN;
+  g.m = g.n = NN;
+  g.k = plan->Ns;
+  g.batch = plan->N;
+  g.alpha = make_float2(1.0f / (float)plan->Ns, 0.f);  // MDT_OneStation_client.cu:683
+  g.beta = make_float2(0.f, 0.f);
+  g.Abase = g.Bbase = (const float2 *)XInputNative;
+  g.Abatch = g.Bbatch = (long)NN * plan->Ns;
+  g.ldI = g.ldJ = plan->Ns;
+  g.Cbase = (float2 *)Vis;
+  g.Cbatch = (long)NN * (NN + 1) / 2;
+  g.herm = 1;
+  g.acc64 = Acc64;
+  g.prec = plan->xprec;
+  g.in_scale = plan->f16_scale;
+  if (plan->xprec == MDT_XPREC_FP16_SPLIT) return xeng_launch_f16split(g, plan->stream);
+  return xeng_launch(g, XL_MK, XO_TRI, plan->stream);
+}
+
+int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative,
+                mdt_complex *Vis, double *Acc64) {
+  MDT_TRY(mdt_fx_fengine(plan, InSignal, XInputNative));
+  return mdt_fx_xengine(plan, XInputNative, Vis, Acc64);
+}
+
+int mdt_fx_plan_destroy(mdt_fx_plan plan) {
+  if (!plan) return MDT_SUCCESS;
+  if (plan->W) (void)hipFree(plan->W);
+  if (plan->tw) (void)hipFree(plan->tw);
+  delete plan;
+  return MDT_SUCCESS;
+}
+
+}  // extern "C"

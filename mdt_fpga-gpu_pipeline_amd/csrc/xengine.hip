@@ -1,0 +1,331 @@
+// xengine.hip -- a5: the X-engine, drop-in for cublasCreate / cublasSetStream
+// / cublasCgemmBatched(OP_N, OP_C, NN, NN, Ns, 1/Ns, A[], NN, A[], NN, 0, C[],
+// NN, N) (MDT_OneStation_client.cu:656-705, 1128), on FP32 MFMA.
+//
+// Complex products on the real v_mfma_f32_32x32x2f32: one MFMA consumes one
+// complex sample as its K = 2 (re, im) pair.  With I-operand rows x_i and
+// J-operand columns x_j,
+//   D_re[i][j] = sum_s (re_i re_j + im_i im_j)          = Re(x_j conj(x_i))
+//   D_im[i][j] = sum_s ((-im_i) re_j + re_i im_j)       = Im(x_j conj(x_i))
+// i.e. D[i][j] = V[j][i] with V[p][q] = sum_s x_p conj(x_q); the I operand is
+// fed as (-im, re) for the second product.  No 3M/Gauss trick: the four real
+// products keep FP32 accuracy.
+//
+// A workgroup computes a 64 x 64 complex tile with 4 waves (2 x 2 MFMA tiles
+// of 32 x 32), streaming K in chunks of KC samples through LDS stored as
+// planar [s][re|im][row] so each MFMA operand read is 32 consecutive floats.
+// Hermitian products (A == B) visit only tiles on or below the diagonal.
+// Workgroups of one batch entry are placed on the same XCD so their shared
+// operand rows hit the same L2.
+#include <new>
+
+#include "common.hpp"
+#include "xengine.hpp"
+
+namespace mdt {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int XT = 64;        // tile edge (complex rows)
+constexpr int XKC = 32;       // K samples per LDS chunk
+constexpr int XRL = XT + 1;   // padded plane row length (floats)
+constexpr int XPLANE = 2 * XRL;  // floats per sample: re row + im row
+
+// Load a 64-row x KC-sample operand chunk into LDS [s][comp][row].
+// KM: element (row, s) at base[row + s*ld] (column-major, rows contiguous).
+template <int LAYOUT>
+__device__ __forceinline__ void load_chunk(float *lds, const float2 *__restrict__ base, long ld,
+                                           int row0, int nrows, int s0, int K) {
+  const int tid = threadIdx.x;
+  if constexpr (LAYOUT == XL_KM) {
+    const int r = tid & 63, sq = tid >> 6;  // 64 rows x 4 sample lanes
+#pragma unroll
+    for (int q = 0; q < XKC / 4; q++) {
+      const int s = sq + 4 * q;
+      float2 v = make_float2(0.f, 0.f);
+      if (row0 + r < nrows && s0 + s < K) v = base[(long)(row0 + r) + (long)(s0 + s) * ld];
+      lds[s * XPLANE + r] = v.x;
+      lds[s * XPLANE + XRL + r] = v.y;
+    }
+  } else {
+    // MK: element (row, s) at base[row*ld + s] (samples contiguous); 16 lanes
+    // x 16 bytes cover 32 samples of one row.
+    const int sp = tid & 15, rq = tid >> 4;  // 16 sample pairs x 16 rows
+#pragma unroll
+    for (int q = 0; q < XT / 16; q++) {
+      const int r = rq + 16 * q;
+      const int s = 2 * sp;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      const long grow = row0 + r;
+      if (grow < nrows) {
+        const float2 *p = base + grow * ld + s0 + s;
+        if (s0 + s + 1 < K && (((uintptr_t)p & 15) == 0)) {
+          v = *(const float4 *)p;
+        } else {
+          if (s0 + s < K) {
+            float2 a = p[0];
+            v.x = a.x;
+            v.y = a.y;
+          }
+          if (s0 + s + 1 < K) {
+            float2 b = p[1];
+            v.z = b.x;
+            v.w = b.y;
+          }
+        }
+      }
+      lds[s * XPLANE + r] = v.x;
+      lds[s * XPLANE + XRL + r] = v.y;
+      lds[(s + 1) * XPLANE + r] = v.z;
+      lds[(s + 1) * XPLANE + XRL + r] = v.w;
+    }
+  }
+}
+
+__device__ __forceinline__ long tri_bl(int a, int b) { return (long)a * (a + 1) / 2 + b; }
+
+// grid.x = tiles_per_batch * batch (XCD-remapped); block = 256 (4 waves)
+template <int LAYOUT, int OUT>
+__global__ __launch_bounds__(256) void xeng_f32_kernel(XArgs g) {
+  __shared__ float ldsI[XKC * XPLANE];
+  __shared__ float ldsJ[XKC * XPLANE];
+
+  // XCD-aware remap: hardware places workgroup w on XCD w % 8; give each XCD a
+  // contiguous range of logical tiles so one batch entry stays on one XCD.
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  long logical = w;
+  if (nwg % 8 == 0) logical = (w % 8) * (nwg / 8) + w / 8;
+  const int batch = (int)(logical / g.tiles);
+  int tile = (int)(logical % g.tiles);
+  int P, Q;  // block indices, P >= Q in Hermitian mode
+  if (g.herm) {
+    P = 0;
+    while ((P + 1) * (P + 2) / 2 <= tile) P++;
+    Q = tile - P * (P + 1) / 2;
+  } else {
+    P = tile / g.nbJ;
+    Q = tile % g.nbJ;
+  }
+  // I-operand block and J-operand block
+  int bi, bj;
+  if (OUT == XO_TRI) {  // i <-> a (row of triangle), j <-> b, a >= b
+    bi = P;
+    bj = Q;
+  } else if (g.herm) {  // j <-> a, i <-> b, a >= b
+    bi = Q;
+    bj = P;
+  } else {
+    bi = P;  // i <-> B rows (n), j <-> A rows (m)
+    bj = Q;
+  }
+  const float2 *baseI, *baseJ;
+  if (OUT == XO_TRI) {
+    baseI = g.Abase + (long)batch * g.Abatch;
+    baseJ = baseI;
+  } else {
+    baseJ = g.Atab ? g.Atab[batch] : g.Abase + (long)batch * g.Abatch;
+    baseI = g.Btab ? g.Btab[batch] : g.Bbase + (long)batch * g.Bbatch;
+  }
+  const int nI = (OUT == XO_TRI) ? g.m : g.n;  // rows available in the I operand
+  const int nJ = g.m;
+  const bool same = g.herm && (bi == bj);
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  // sub-tile strictly on the excluded side of the diagonal: skip the MFMAs
+  bool skip = false;
+  if (same) skip = (OUT == XO_TRI) ? (wi < wj) : (wj < wi);
+
+  f32x16 accR, accI;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    accR[r] = 0.f;
+    accI[r] = 0.f;
+  }
+  const int lrow = lane & 31;
+  const int comp = lane >> 5;  // 0: re, 1: im
+  const int rowI = wi * 32 + lrow, rowJ = wj * 32 + lrow;
+  const float *srcJ = same ? ldsI : ldsJ;
+
+  for (int s0 = 0; s0 < g.k; s0 += XKC) {
+    __syncthreads();
+    load_chunk<LAYOUT>(ldsI, baseI, g.ldI, bi * XT, nI, s0, g.k);
+    if (!same) load_chunk<LAYOUT>(ldsJ, baseJ, g.ldJ, bj * XT, nJ, s0, g.k);
+    __syncthreads();
+    if (!skip) {
+#pragma unroll 8
+      for (int s = 0; s < XKC; s++) {
+        const float *pI = ldsI + s * XPLANE;
+        const float *pJ = srcJ + s * XPLANE;
+        float vi = pI[comp * XRL + rowI];
+        float vit = pI[(1 - comp) * XRL + rowI];
+        vit = comp ? vit : -vit;  // (-im, re)
+        float vj = pJ[comp * XRL + rowJ];
+        accR = __builtin_amdgcn_mfma_f32_32x32x2f32(vi, vj, accR, 0, 0, 0);
+        accI = __builtin_amdgcn_mfma_f32_32x32x2f32(vit, vj, accI, 0, 0, 0);
+      }
+    }
+  }
+  if (skip) return;
+
+  // D[i][j] = V[j][i]; lane holds column j = lane & 31, rows
+  // i = 8*(r/4) + 4*(lane/32) + r%4.
+  const int jg = bj * XT + wj * 32 + lrow;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int ig = bi * XT + wi * 32 + 8 * (r >> 2) + 4 * comp + (r & 3);
+    const float dre = accR[r], dim = accI[r];
+    if constexpr (OUT == XO_TRI) {
+      // a = ig, b = jg, V[a][b] = conj(D)
+      const int a = ig, b = jg;
+      if (a < g.m && b < g.m && a >= b) {
+        const long o = (long)batch * g.Cbatch + tri_bl(a, b);
+        const float vr = g.alpha.x * dre, vim = -g.alpha.x * dim;
+        g.Cbase[o] = make_float2(vr, vim);
+        if (g.acc64) {
+          double2 *acc = (double2 *)g.acc64;
+          double2 t = acc[o];
+          t.x += (double)vr;
+          t.y += (double)vim;
+          acc[o] = t;
+        }
+      }
+    } else {
+      // a = jg (row of C), b = ig (column of C), V[a][b] = D
+      const int a = jg, b = ig;
+      if (a >= g.m || b >= g.n) continue;
+      if (g.herm && a < b) continue;
+      float2 *C = g.Ctab ? g.Ctab[batch] : g.Cbase + (long)batch * g.Cbatch;
+      const float2 al = g.alpha, be = g.beta;
+      {
+        float2 c0 = (be.x != 0.f || be.y != 0.f) ? C[a + (long)b * g.ldc] : make_float2(0.f, 0.f);
+        float2 o = make_float2(al.x * dre - al.y * dim + be.x * c0.x - be.y * c0.y,
+                               al.x * dim + al.y * dre + be.x * c0.y + be.y * c0.x);
+        C[a + (long)b * g.ldc] = o;
+      }
+      if (g.herm && a != b) {  // mirror: V[b][a] = conj(V[a][b])
+        float2 c0 = (be.x != 0.f || be.y != 0.f) ? C[b + (long)a * g.ldc] : make_float2(0.f, 0.f);
+        float2 o = make_float2(al.x * dre + al.y * dim + be.x * c0.x - be.y * c0.y,
+                               -al.x * dim + al.y * dre + be.x * c0.y + be.y * c0.x);
+        C[b + (long)a * g.ldc] = o;
+      }
+    }
+  }
+}
+
+int xeng_launch(const XArgs &g0, int layout, int out, hipStream_t st) {
+  XArgs g = g0;
+  const int nbM = (g.m + XT - 1) / XT;
+  const int nbN = ((out == XO_TRI ? g.m : g.n) + XT - 1) / XT;
+  if (g.herm) {
+    g.tiles = nbM * (nbM + 1) / 2;
+    g.nbJ = nbM;
+  } else {
+    g.tiles = nbN * nbM;  // P over I blocks (n), Q over J blocks (m)
+    g.nbJ = nbM;
+  }
+  long total = (long)g.tiles * g.batch;
+  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "X-engine grid too large");
+  dim3 grid((unsigned)total), block(256);
+  if (layout == XL_KM && out == XO_FULL)
+    hipLaunchKernelGGL((xeng_f32_kernel<XL_KM, XO_FULL>), grid, block, 0, st, g);
+  else if (layout == XL_KM && out == XO_TRI)
+    hipLaunchKernelGGL((xeng_f32_kernel<XL_KM, XO_TRI>), grid, block, 0, st, g);
+  else if (layout == XL_MK && out == XO_TRI)
+    hipLaunchKernelGGL((xeng_f32_kernel<XL_MK, XO_TRI>), grid, block, 0, st, g);
+  else
+    hipLaunchKernelGGL((xeng_f32_kernel<XL_MK, XO_FULL>), grid, block, 0, st, g);
+  return check_launch("xeng_f32_kernel");
+}
+
+}  // namespace mdt
+
+struct mdt_xengine_s {
+  hipStream_t stream = 0;
+};
+
+using namespace mdt;
+
+extern "C" {
+
+int mdt_xengine_create(mdt_xengine *handle) {
+  MDT_REQUIRE_PTR(handle);
+  *handle = new (std::nothrow) mdt_xengine_s;
+  if (!*handle) return fail(MDT_ERR_ALLOC, "X-engine handle");
+  return MDT_SUCCESS;
+}
+
+int mdt_xengine_set_stream(mdt_xengine handle, hipStream_t stream) {
+  MDT_REQUIRE_PTR(handle);
+  handle->stream = stream;
+  return MDT_SUCCESS;
+}
+
+int mdt_xengine_destroy(mdt_xengine handle) {
+  delete handle;
+  return MDT_SUCCESS;
+}
+
+int mdt_cgemm_batched(mdt_xengine handle, int transa, int transb, int m, int n, int k,
+                      const mdt_complex *alpha, const mdt_complex *const Aarray[], int lda,
+                      const mdt_complex *const Barray[], int ldb, const mdt_complex *beta,
+                      mdt_complex *const Carray[], int ldc, int batchCount) {
+  MDT_REQUIRE_PTR(handle);
+  MDT_REQUIRE_PTR(alpha);
+  MDT_REQUIRE_PTR(beta);
+  MDT_REQUIRE_PTR(Aarray);
+  MDT_REQUIRE_PTR(Barray);
+  MDT_REQUIRE_PTR(Carray);
+  if (transa != MDT_OP_N || transb != MDT_OP_C)
+    return fail(MDT_ERR_UNSUPPORTED, "only transa=N, transb=C (the X-engine call) is provided");
+  if (m < 0 || n < 0 || k < 0 || batchCount < 0)
+    return fail(MDT_ERR_INVALID_VALUE, "negative dimension");
+  if (lda < (m > 1 ? m : 1) || ldb < (n > 1 ? n : 1) || ldc < (m > 1 ? m : 1))
+    return fail(MDT_ERR_INVALID_VALUE, "leading dimension too small (lda>=m, ldb>=n, ldc>=m)");
+  if (m == 0 || n == 0 || batchCount == 0) return MDT_SUCCESS;
+  XArgs g{};
+  g.m = m;
+  g.n = n;
+  g.k = k;
+  g.batch = batchCount;
+  g.alpha = make_float2(alpha->x, alpha->y);
+  g.beta = make_float2(beta->x, beta->y);
+  g.Atab = (const float2 *const *)Aarray;
+  g.Btab = (const float2 *const *)Barray;
+  g.Ctab = (float2 *const *)Carray;
+  g.ldJ = lda;  // J operand = A (rows a), column-major: element (row, s) at row + s*lda
+  g.ldI = ldb;
+  g.ldc = ldc;
+  g.herm = ((const void *)Aarray == (const void *)Barray) && m == n && lda == ldb;
+  return xeng_launch(g, XL_KM, XO_FULL, handle->stream);
+}
+
+int mdt_xengine_herk(mdt_xengine handle, int NpolsxNelements, int Nspectra, float alpha,
+                     const mdt_complex *XInput, mdt_complex *XOutput, int Nchannels) {
+  MDT_REQUIRE_PTR(handle);
+  MDT_REQUIRE_PTR(XInput);
+  MDT_REQUIRE_PTR(XOutput);
+  MDT_TRY(check_inputs(NpolsxNelements));
+  MDT_TRY(check_spectra(Nspectra));
+  MDT_TRY(check_channels(Nchannels));
+  XArgs g{};
+  const int NN = NpolsxNelements;
+  g.m = g.n = NN;
+  g.k = Nspectra;
+  g.batch = Nchannels;
+  g.alpha = make_float2(alpha, 0.f);
+  g.beta = make_float2(0.f, 0.f);
+  g.Abase = g.Bbase = (const float2 *)XInput;
+  g.Abatch = g.Bbatch = (long)NN * Nspectra;
+  g.ldI = g.ldJ = NN;
+  g.Cbase = (float2 *)XOutput;
+  g.Cbatch = (long)NN * NN;
+  g.ldc = NN;
+  g.herm = 1;
+  return xeng_launch(g, XL_KM, XO_FULL, handle->stream);
+}
+
+}  // extern "C"
+

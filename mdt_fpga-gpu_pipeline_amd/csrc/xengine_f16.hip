@@ -1,0 +1,239 @@
+// xengine_f16.hip -- the split-precision X-engine of the native path on
+// v_mfma_f32_32x32x16_f16: each FP32 sample x (pre-scaled by an exact power
+// of two) is split into hi = fp16(x) and lo = fp16(x - hi), and
+//   x_i * x_j ~= hi_i hi_j + hi_i lo_j + lo_i hi_j        (FP32 accumulate)
+// which keeps ~22 significant bits per operand; the dropped lo_i lo_j term is
+// below 2^-22 relative.  Same complex scheme as the FP32 kernel: K runs over
+// interleaved (re, im) halves, the I operand is fed as (-im, re) for the
+// imaginary part, D[i][j] = V[j][i].
+//
+// Workgroup tile 128 x 128 complex (4 waves of 64 x 64 = 2 x 2 MFMA tiles),
+// K streamed in chunks of 16 samples; the next chunk is prefetched into
+// registers while the current one is multiplied out of LDS.  FP32 -> hi/lo
+// conversion happens once per element on the way into LDS.  Only tiles on or
+// below the diagonal are visited and 32 x 32 MFMA tiles strictly above it are
+// skipped.  Output: packed lower triangle, optional FP64 accumulation.
+#include "common.hpp"
+#include "xengine.hpp"
+
+namespace mdt {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int HT = 128;               // workgroup tile edge (complex rows)
+constexpr int HKC = 16;               // samples per K chunk
+constexpr int HROWB = HKC * 4 + 16;   // bytes per LDS row per plane (64 data + 16 pad)
+constexpr int HPLANE = HT * HROWB;    // bytes per plane
+
+__device__ __forceinline__ unsigned pack_h2(_Float16 a, _Float16 b) {
+  return (unsigned)__builtin_bit_cast(unsigned short, a) |
+         ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
+// hi/lo split of two complex samples (4 floats) scaled by an exact power of 2
+__device__ __forceinline__ void split4(float4 v, float sc, uint2 &hi, uint2 &lo) {
+  float x[4] = {v.x * sc, v.y * sc, v.z * sc, v.w * sc};
+  _Float16 h[4], l[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    h[q] = (_Float16)x[q];
+    l[q] = (_Float16)(x[q] - (float)h[q]);
+  }
+  hi = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
+  lo = make_uint2(pack_h2(l[0], l[1]), pack_h2(l[2], l[3]));
+}
+
+// (re, im) half pairs -> (-im, re)
+__device__ __forceinline__ half8 rot_conj(half8 v) {
+  u32x4 u = __builtin_bit_cast(u32x4, v);
+#pragma unroll
+  for (int q = 0; q < 4; q++) u[q] = __builtin_amdgcn_alignbit(u[q], u[q], 16) ^ 0x8000u;
+  return __builtin_bit_cast(half8, u);
+}
+
+struct Prefetch {
+  float4 v[4];
+};
+
+// global -> registers: 128 rows x 16 samples (8 lanes x 16 B per row)
+__device__ __forceinline__ void fetch(Prefetch &p, const float2 *__restrict__ base, long ld,
+                                      int row0, int nrows, int s0, int K) {
+  const int seg = threadIdx.x & 7, rq = threadIdx.x >> 3;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int row = rq + 32 * q;
+    const long grow = row0 + row;
+    const int s = s0 + 2 * seg;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (grow < nrows) {
+      const float2 *src = base + grow * ld + s;
+      if (s + 1 < K && (((uintptr_t)src & 15) == 0)) {
+        v = *(const float4 *)src;
+      } else {
+        if (s < K) {
+          float2 a = src[0];
+          v.x = a.x;
+          v.y = a.y;
+        }
+        if (s + 1 < K) {
+          float2 b = src[1];
+          v.z = b.x;
+          v.w = b.y;
+        }
+      }
+    }
+    p.v[q] = v;
+  }
+}
+
+// registers -> LDS hi/lo planes, [row][k] with k = 2*s + (re|im)
+__device__ __forceinline__ void stage(const Prefetch &p, char *lds, float sc) {
+  const int seg = threadIdx.x & 7, rq = threadIdx.x >> 3;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int row = rq + 32 * q;
+    uint2 hi, lo;
+    split4(p.v[q], sc, hi, lo);
+    *(uint2 *)(lds + row * HROWB + seg * 8) = hi;
+    *(uint2 *)(lds + HPLANE + row * HROWB + seg * 8) = lo;
+  }
+}
+
+__device__ __forceinline__ long tri_bl16(int a, int b) { return (long)a * (a + 1) / 2 + b; }
+
+__global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float sc, float oscale) {
+  __shared__ __attribute__((aligned(16))) char lds[2 * 2 * HPLANE];  // [op I|J][hi|lo][row][k]
+
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+  const int batch = (int)(logical / g.tiles);
+  const int tile = (int)(logical % g.tiles);
+  int P = 0;
+  while ((P + 1) * (P + 2) / 2 <= tile) P++;
+  const int Q = tile - P * (P + 1) / 2;
+  const int bi = P, bj = Q;  // I rows <-> a, J rows <-> b, a >= b
+  const bool same = (bi == bj);
+  const float2 *base = g.Abase + (long)batch * g.Abatch;
+  const int NN = g.m, K = g.k;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int l32 = lane & 31, kh = lane >> 5;
+  // 32-row block indices of this wave's 2 x 2 MFMA tiles
+  bool act[2][2];
+  bool any = false;
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int ra = bi * 4 + wi * 2 + t, rb = bj * 4 + wj * 2 + u;
+      act[t][u] = (ra >= rb) && (ra * 32 < NN) && (rb * 32 < NN);
+      any |= act[t][u];
+    }
+
+  f32x16 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        accR[t][u][r] = 0.f;
+        accI[t][u][r] = 0.f;
+      }
+
+  char *ldsI = lds;
+  char *ldsJ = same ? lds : lds + 2 * HPLANE;
+  Prefetch pI, pJ;
+  fetch(pI, base, g.ldI, bi * HT, NN, 0, K);
+  if (!same) fetch(pJ, base, g.ldJ, bj * HT, NN, 0, K);
+
+  for (int s0 = 0; s0 < K; s0 += HKC) {
+    __syncthreads();
+    stage(pI, lds, sc);
+    if (!same) stage(pJ, lds + 2 * HPLANE, sc);
+    __syncthreads();
+    if (s0 + HKC < K) {  // prefetch the next chunk while this one is multiplied
+      fetch(pI, base, g.ldI, bi * HT, NN, s0 + HKC, K);
+      if (!same) fetch(pJ, base, g.ldJ, bj * HT, NN, s0 + HKC, K);
+    }
+    if (any) {
+#pragma unroll
+      for (int ks = 0; ks < HKC / 8; ks++) {
+        const int koff = ks * 32 + kh * 16;  // bytes: 8 samples = 16 halves per k-step
+        half8 Ih[2], Il[2], Jh[2], Jl[2], Ph[2], Pl[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          const int row = wi * 64 + t * 32 + l32;
+          Ih[t] = *(const half8 *)(ldsI + row * HROWB + koff);
+          Il[t] = *(const half8 *)(ldsI + HPLANE + row * HROWB + koff);
+          Ph[t] = rot_conj(Ih[t]);
+          Pl[t] = rot_conj(Il[t]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int row = wj * 64 + u * 32 + l32;
+          Jh[u] = *(const half8 *)(ldsJ + row * HROWB + koff);
+          Jl[u] = *(const half8 *)(ldsJ + HPLANE + row * HROWB + koff);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            if (!act[t][u]) continue;
+            accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih[t], Jh[u], accR[t][u], 0, 0, 0);
+            accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph[t], Jh[u], accI[t][u], 0, 0, 0);
+            accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih[t], Jl[u], accR[t][u], 0, 0, 0);
+            accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph[t], Jl[u], accI[t][u], 0, 0, 0);
+            accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il[t], Jh[u], accR[t][u], 0, 0, 0);
+            accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl[t], Jh[u], accI[t][u], 0, 0, 0);
+          }
+      }
+    }
+  }
+  if (!any) return;
+
+  // D[i][j] = V[b = j][a = i]; V[a][b] = conj(D) * alpha / sc^2
+  double2 *acc = (double2 *)g.acc64;
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (!act[t][u]) continue;
+      const int b = bj * HT + wj * 64 + u * 32 + l32;
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int a = bi * HT + wi * 64 + t * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+        if (a < NN && b < NN && a >= b) {
+          const long o = (long)batch * g.Cbatch + tri_bl16(a, b);
+          const float vr = oscale * accR[t][u][r], vi = -oscale * accI[t][u][r];
+          g.Cbase[o] = make_float2(vr, vi);
+          if (acc) {
+            double2 x = acc[o];
+            x.x += (double)vr;
+            x.y += (double)vi;
+            acc[o] = x;
+          }
+        }
+      }
+    }
+}
+
+int xeng_launch_f16split(const XArgs &g0, hipStream_t st) {
+  XArgs g = g0;
+  const int nb = (g.m + HT - 1) / HT;
+  g.tiles = nb * (nb + 1) / 2;
+  long total = (long)g.tiles * g.batch;
+  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "X-engine grid too large");
+  // power-of-two input scale (exact) chosen by the caller; products scale by sc^2
+  const float sc = g.in_scale;
+  if (!(sc > 0.f)) return fail(MDT_ERR_INVALID_VALUE, "FP16 split needs a positive input scale");
+  const float oscale = g.alpha.x / (sc * sc);
+  hipLaunchKernelGGL(xeng_f16split_kernel, dim3((unsigned)total), dim3(256), 0, st, g, sc, oscale);
+  return check_launch("xeng_f16split_kernel");
+}
+
+}  // namespace mdt

@@ -1,0 +1,328 @@
+"""Python front end of libmdtfx (the MI355X-native MDT FX correlator).
+
+Thin ctypes bindings over the C ABI declared in include/mdtfx.h.  PyTorch is
+used only for device memory and the current stream: every call takes torch
+tensors that already live on the GPU and passes their raw pointers, in the
+reference's argument order, to the HIP library.  There is no CPU fallback: a
+missing library or a CPU tensor raises immediately.
+
+Mirrors of the reference's GPU call surface (MDT_OneStation_client.cu):
+  pfb_window        PFB_Window_min4term_BH   (MDT_device_functions.cu:52-88)
+  pps_batch[_i8]    PpS_Batch                (MDT_device_functions.cu:113-161)
+  FftPlan           cufftPlan1d/SetStream/ExecC2C (client.cu:622, 630, 1105)
+  reorder_foutput   ReorderFOutput           (MDT_device_functions.cu:183-208)
+  XEngine           cublasCreate/SetStream/CgemmBatched (client.cu:690-705, 1128)
+  history_shift     the (T-1)-spectrum carry (client.cu:1170)
+  unpack_u32        SNAP word decode          (client.cu:898-907)
+  FxPlan            native fused path (int8 -> triangle visibilities)
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_PKG, "libmdtfx.so")
+
+SUCCESS = 0
+ERR_INVALID_VALUE = 1
+ERR_NULL_POINTER = 2
+ERR_UNSUPPORTED = 3
+ERR_LAUNCH = 4
+ERR_ALLOC = 5
+ERR_DEVICE = 6
+ERR_MISALIGNED = 7
+
+WINDOW_BH4 = 0
+WINDOW_HAMMING = 1
+FFT_FORWARD = -1
+FFT_INVERSE = 1
+FFT_C2C = 0x29
+OP_N, OP_T, OP_C = 0, 1, 2
+XPREC_FP32 = 0
+XPREC_FP16_SPLIT = 1
+
+_lib = None
+
+
+class MdtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mdtfx error {code}: {msg}")
+        self.code = code
+
+
+class _Complex(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float)]
+
+
+class SynthParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("sigma", ctypes.c_float), ("rho", ctypes.c_float),
+                ("max_delay", ctypes.c_int), ("ntones", ctypes.c_int),
+                ("tone_freq", ctypes.c_float * 4), ("tone_amp", ctypes.c_float * 4),
+                ("t0", ctypes.c_longlong)]
+
+
+# name -> argtypes (all return int status)
+_VP, _I, _F, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+_SIGS = {
+    "mdt_version": [ctypes.POINTER(ctypes.c_int)],
+    "mdt_pfb_window": [_I, _I, _VP, _I, _VP],
+    "mdt_pps_batch": [_I, _I, _VP, _VP, _VP, _I, _I, _VP],
+    "mdt_pps_batch_i8": [_I, _I, _VP, _VP, _VP, _I, _I, _VP],
+    "mdt_fft_plan_1d": [ctypes.POINTER(_VP), _I, _I, _I],
+    "mdt_fft_set_stream": [_VP, _VP],
+    "mdt_fft_exec_c2c": [_VP, _VP, _VP, _I],
+    "mdt_fft_destroy": [_VP],
+    "mdt_reorder_foutput": [_I, _I, _VP, _VP, _I, _VP],
+    "mdt_xengine_create": [ctypes.POINTER(_VP)],
+    "mdt_xengine_set_stream": [_VP, _VP],
+    "mdt_xengine_destroy": [_VP],
+    "mdt_cgemm_batched": [_VP, _I, _I, _I, _I, _I, ctypes.POINTER(_Complex), _VP, _I, _VP, _I,
+                          ctypes.POINTER(_Complex), _VP, _I, _I],
+    "mdt_xengine_herk": [_VP, _I, _I, _F, _VP, _VP, _I],
+    "mdt_history_shift": [_VP, _I, _I, _I, _I, _I, _VP],
+    "mdt_unpack_u32": [_VP, _VP, _SZ, _VP],
+    "mdt_fx_plan_create": [ctypes.POINTER(_VP), _I, _I, _I, _I, _I, _I],
+    "mdt_fx_set_stream": [_VP, _VP],
+    "mdt_fx_set_precision": [_VP, _I],
+    "mdt_fx_xinput_bytes": [_VP, ctypes.POINTER(_SZ)],
+    "mdt_fx_get_window": [_VP, _VP],
+    "mdt_fx_fengine": [_VP, _VP, _VP],
+    "mdt_fx_xengine": [_VP, _VP, _VP, _VP],
+    "mdt_fx_dump": [_VP, _VP, _VP, _VP, _VP],
+    "mdt_fx_plan_destroy": [_VP],
+    "mdt_synth_i8": [ctypes.POINTER(SynthParams), _I, _I, _I, _VP, _VP],
+}
+
+
+def lib():
+    """Load libmdtfx.so (built in-tree by __graft_entry__.build()); no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libmdtfx.so not found at {LIB_PATH}: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.mdt_last_error_string.restype = ctypes.c_char_p
+        L.mdt_last_error_string.argtypes = []
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != SUCCESS:
+        raise MdtError(status, lib().mdt_last_error_string().decode())
+    return status
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t, dtype=None):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("mdtfx works on GPU tensors only (no CPU fallback)")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def version():
+    v = ctypes.c_int()
+    check(lib().mdt_version(ctypes.byref(v)))
+    return v.value
+
+
+# ------------------------------------------------------------------ a1
+def pfb_window(N, T, kind=WINDOW_BH4, out=None):
+    if out is None:
+        out = torch.empty((T, N), dtype=torch.float32, device="cuda")
+    check(lib().mdt_pfb_window(N, T, _ptr(out, torch.float32), kind, _stream()))
+    return out
+
+
+# ------------------------------------------------------------------ a2
+def pps_batch(W, x, NN, Ns, out=None):
+    """x: complex64 [(Ns+T-1), NN, N]."""
+    T, N = W.shape
+    if out is None:
+        out = torch.empty((Ns, NN, N), dtype=torch.complex64, device="cuda")
+    check(lib().mdt_pps_batch(N, T, _ptr(W, torch.float32), _ptr(x, torch.complex64),
+                              _ptr(out, torch.complex64), NN, Ns, _stream()))
+    return out
+
+
+def pps_batch_i8(W, x8, NN, Ns, out=None):
+    """x8: int8 [(Ns+T-1), NN, N, 2]."""
+    T, N = W.shape
+    if out is None:
+        out = torch.empty((Ns, NN, N), dtype=torch.complex64, device="cuda")
+    check(lib().mdt_pps_batch_i8(N, T, _ptr(W, torch.float32), _ptr(x8, torch.int8),
+                                 _ptr(out, torch.complex64), NN, Ns, _stream()))
+    return out
+
+
+# ------------------------------------------------------------------ a3
+class FftPlan:
+    """cufftPlan1d(nx, CUFFT_C2C, batch) equivalent."""
+
+    def __init__(self, nx, batch, type=FFT_C2C):
+        self._h = ctypes.c_void_p()
+        check(lib().mdt_fft_plan_1d(ctypes.byref(self._h), nx, type, batch))
+        self.nx, self.batch = nx, batch
+
+    def set_stream(self, stream_handle):
+        check(lib().mdt_fft_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def exec_c2c(self, idata, odata, direction=FFT_FORWARD):
+        check(lib().mdt_fft_exec_c2c(self._h, _ptr(idata, torch.complex64),
+                                     _ptr(odata, torch.complex64), direction))
+        return odata
+
+    def destroy(self):
+        if self._h:
+            check(lib().mdt_fft_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ a4
+def reorder_foutput(F, NN, N, Ns, out=None):
+    if out is None:
+        out = torch.empty((N, Ns, NN), dtype=F.dtype, device="cuda")
+    check(lib().mdt_reorder_foutput(NN, N, _ptr(F), _ptr(out), Ns, _stream()))
+    return out
+
+
+# ------------------------------------------------------------------ a5
+class XEngine:
+    """cublasCreate + cublasSetStream + cublasCgemmBatched equivalent."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().mdt_xengine_create(ctypes.byref(self._h)))
+
+    def set_stream(self, stream_handle):
+        check(lib().mdt_xengine_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def cgemm_batched(self, transa, transb, m, n, k, alpha, Aptrs, lda, Bptrs, ldb, beta,
+                      Cptrs, ldc, batch):
+        """A/B/C pointer tables: int64 device tensors of data_ptr() values."""
+        al = _Complex(float(complex(alpha).real), float(complex(alpha).imag))
+        be = _Complex(float(complex(beta).real), float(complex(beta).imag))
+        check(lib().mdt_cgemm_batched(self._h, transa, transb, m, n, k, ctypes.byref(al),
+                                      _ptr(Aptrs, torch.int64), lda, _ptr(Bptrs, torch.int64), ldb,
+                                      ctypes.byref(be), _ptr(Cptrs, torch.int64), ldc, batch))
+
+    def herk(self, NN, Ns, alpha, XIn, XOut, N):
+        check(lib().mdt_xengine_herk(self._h, NN, Ns, float(alpha), _ptr(XIn, torch.complex64),
+                                     _ptr(XOut, torch.complex64), N))
+        return XOut
+
+    def destroy(self):
+        if self._h:
+            check(lib().mdt_xengine_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ a6, a7
+def history_shift(FInput, NN, N, T, Ns):
+    eb = 8 if FInput.dtype == torch.complex64 else 2
+    check(lib().mdt_history_shift(_ptr(FInput), NN, N, T, Ns, eb, _stream()))
+
+
+def unpack_u32(words, out=None):
+    """words: int32 tensor holding the raw uint32 bit patterns."""
+    if out is None:
+        out = torch.empty(words.shape, dtype=torch.float32, device="cuda")
+    check(lib().mdt_unpack_u32(_ptr(words), _ptr(out, torch.float32), words.numel(), _stream()))
+    return out
+
+
+# ------------------------------------------------------------------ native path
+class FxPlan:
+    """One-dump plan of the fused path: int8 [Ns+T-1][NN][N][2] ->
+    X input [N][NN][Ns] complex64 -> triangle visibilities [N][NN(NN+1)/2]."""
+
+    def __init__(self, NN, N, T, Ns, window=WINDOW_BH4, xprec=XPREC_FP32):
+        self._h = ctypes.c_void_p()
+        check(lib().mdt_fx_plan_create(ctypes.byref(self._h), NN, N, T, Ns, window, xprec))
+        self.NN, self.N, self.T, self.Ns = NN, N, T, Ns
+        self.nb = NN * (NN + 1) // 2
+
+    def set_precision(self, xprec):
+        check(lib().mdt_fx_set_precision(self._h, xprec))
+
+    def set_stream(self, stream_handle):
+        check(lib().mdt_fx_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def window(self):
+        W = torch.empty((self.T, self.N), dtype=torch.float32, device="cuda")
+        check(lib().mdt_fx_get_window(self._h, _ptr(W, torch.float32)))
+        return W
+
+    def alloc_xinput(self):
+        return torch.empty((self.N, self.NN, self.Ns), dtype=torch.complex64, device="cuda")
+
+    def alloc_vis(self):
+        return torch.empty((self.N, self.nb), dtype=torch.complex64, device="cuda")
+
+    def fengine(self, x8, xin):
+        check(lib().mdt_fx_fengine(self._h, _ptr(x8, torch.int8), _ptr(xin, torch.complex64)))
+        return xin
+
+    def xengine(self, xin, vis, acc64=None):
+        check(lib().mdt_fx_xengine(self._h, _ptr(xin, torch.complex64), _ptr(vis, torch.complex64),
+                                   _ptr(acc64, torch.complex128) if acc64 is not None else None))
+        return vis
+
+    def dump(self, x8, xin, vis, acc64=None):
+        check(lib().mdt_fx_dump(self._h, _ptr(x8, torch.int8), _ptr(xin, torch.complex64),
+                                _ptr(vis, torch.complex64),
+                                _ptr(acc64, torch.complex128) if acc64 is not None else None))
+        return vis
+
+    def destroy(self):
+        if self._h:
+            check(lib().mdt_fx_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def synth_i8(NN, N, nframes, seed=1, sigma=16.0, rho=0.0, max_delay=0, tones=(), t0=0, out=None):
+    """Deterministic int8 (I, Q) samples [nframes][NN][N][2] on the GPU."""
+    p = SynthParams()
+    p.seed, p.sigma, p.rho, p.max_delay = seed, sigma, rho, max_delay
+    p.ntones = len(tones)
+    for i, (f, a) in enumerate(tones):
+        p.tone_freq[i] = f
+        p.tone_amp[i] = a
+    p.t0 = t0
+    if out is None:
+        out = torch.empty((nframes, NN, N, 2), dtype=torch.int8, device="cuda")
+    check(lib().mdt_synth_i8(ctypes.byref(p), NN, N, nframes, _ptr(out, torch.int8), _stream()))
+    return out

@@ -1,0 +1,143 @@
+"""GPU parity of the native fused path (int8 -> fused F-engine -> triangle
+X-engine) against the oracle and against the composition of the drop-in
+kernels, called through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+import mdtfx
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _tri_autonorm_err(vis, v64, NN):
+    """max over (c, a >= b) of |V - V64| / sqrt(V64_aa V64_bb)"""
+    a, b = np.tril_indices(NN)
+    diag = np.array([orc.tri_index(i, i) for i in range(NN)])
+    auto = np.real(v64[:, diag])
+    norm = np.sqrt(auto[:, a] * auto[:, b])
+    return float(np.max(np.abs(vis - v64) / norm))
+
+
+@pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (3, 32, 4, 11), (5, 64, 8, 17),
+                                       (4, 2048, 4, 9), (3, 4096, 8, 16), (3, 8192, 4, 5),
+                                       (3, 16384, 4, 3), (6, 256, 5, 12)])
+def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns):
+    """fused PFB+FFT+corner turn == pps_batch_i8 -> fft -> reorder, bit for bit"""
+    print(f"fengine NN={NN} N={N} T={T} Ns={Ns}", flush=True)
+    plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_HAMMING)
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=N + T, sigma=16.0, tones=((0.1234, 20.0),))
+    xin = plan.fengine(x8, plan.alloc_xinput())
+    W = plan.window()
+    F = mdtfx.pps_batch_i8(W, x8, NN, Ns)
+    fp = mdtfx.FftPlan(N, Ns * NN)
+    fp.set_stream(torch.cuda.current_stream().cuda_stream)
+    fp.exec_c2c(F, F)
+    ref = _host(F).transpose(2, 1, 0)  # [Ns][NN][N] -> [N][NN][Ns]
+    got = _host(xin)
+    assert np.array_equal(got.view(np.uint32), np.ascontiguousarray(ref).view(np.uint32))
+
+
+def test_native_chain_c1_vs_fp64():
+    """Operating point C1: 40 inputs, 4-tap Hamming PFB, 1024 channels, 256
+    spectra, int8 N(0, 16^2) + 2 tones -> 820 baselines x 1024 channels."""
+    NN, N, T, Ns = 40, 1024, 4, 256
+    plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_HAMMING)
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=1, sigma=16.0, tones=((0.1, 12.0), (-0.3, 8.0)))
+    xin = plan.alloc_xinput()
+    vis = plan.dump(x8, xin, plan.alloc_vis())
+    V = _host(vis)
+    W = _host(plan.window()).astype(np.float64)
+    v64 = orc.fx_chain_f64(W, _host(x8), NN, N, T, Ns, np.arange(N))
+    err = _tri_autonorm_err(V, v64, NN)
+    print(f"C1 max autocorrelation-normalised error vs FP64: {err:.3e}")
+    assert err < 1e-5
+    # FP32 port on the CPU agrees to the same bar
+    v32 = orc.fx_chain_f32(W.astype(np.float32), _host(x8), NN, N, T, Ns)
+    assert _tri_autonorm_err(v32, v64, NN) < 1e-5
+
+
+@pytest.mark.parametrize("NN,N,T,Ns", [(3, 32, 4, 2), (65, 256, 4, 70), (130, 128, 8, 40)])
+def test_native_xengine_edges(NN, N, T, Ns):
+    """tile edges (NN not a multiple of 64, Ns not a multiple of 32)"""
+    print(f"native NN={NN} N={N} T={T} Ns={Ns}", flush=True)
+    plan = mdtfx.FxPlan(NN, N, T, Ns)
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=NN, sigma=10.0, rho=0.3)
+    vis = plan.dump(x8, plan.alloc_xinput(), torch.full((N, plan.nb), complex(np.nan, np.nan), dtype=torch.complex64, device="cuda"))
+    V = _host(vis)
+    assert np.all(np.isfinite(V))
+    W = _host(plan.window()).astype(np.float64)
+    chans = np.arange(N)
+    v64 = orc.fx_chain_f64(W, _host(x8), NN, N, T, Ns, chans)
+    assert _tri_autonorm_err(V, v64, NN) < 1e-5
+
+
+def test_fp64_accumulator_two_dumps():
+    NN, N, T, Ns = 8, 64, 4, 16
+    plan = mdtfx.FxPlan(NN, N, T, Ns)
+    acc = torch.zeros((N, plan.nb), dtype=torch.complex128, device="cuda")
+    vs = []
+    for d in range(2):
+        x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=10 + d)
+        vs.append(_host(plan.dump(x8, plan.alloc_xinput(), plan.alloc_vis(), acc)).astype(np.complex128))
+    assert np.array_equal(_host(acc), vs[0] + vs[1])
+
+
+def test_streaming_history_equals_long_run():
+    """D dumps with the (T-1)-spectrum carry (client.cu:1170) == one long run"""
+    NN, N, T, Ns, D = 4, 256, 8, 16, 3
+    long_x8 = mdtfx.synth_i8(NN, N, D * Ns + T - 1, seed=77, sigma=12.0)
+    long_plan = mdtfx.FxPlan(NN, N, T, D * Ns)
+    long_xin = _host(long_plan.fengine(long_x8, long_plan.alloc_xinput()))  # [N][NN][D*Ns]
+    plan = mdtfx.FxPlan(NN, N, T, Ns)
+    buf = torch.empty((Ns + T - 1, NN, N, 2), dtype=torch.int8, device="cuda")
+    buf[:T - 1] = long_x8[:T - 1]
+    for d in range(D):
+        buf[T - 1:] = long_x8[T - 1 + d * Ns: T - 1 + (d + 1) * Ns]
+        xin = _host(plan.fengine(buf, plan.alloc_xinput()))
+        assert np.array_equal(xin.view(np.uint32), np.ascontiguousarray(long_xin[:, :, d * Ns:(d + 1) * Ns]).view(np.uint32))
+        mdtfx.history_shift(buf, NN, N, T, Ns)
+
+
+def test_synth_deterministic_and_statistics():
+    a = _host(mdtfx.synth_i8(16, 1024, 32, seed=5, sigma=16.0))
+    b = _host(mdtfx.synth_i8(16, 1024, 32, seed=5, sigma=16.0))
+    c = _host(mdtfx.synth_i8(16, 1024, 32, seed=6, sigma=16.0))
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    s = a.astype(np.float64)
+    assert abs(s.mean()) < 0.2
+    assert abs(s.std() - 16.0 / np.sqrt(2)) < 0.3  # sigma^2 split over I and Q
+    # common component: correlation coefficient rho between inputs
+    r = _host(mdtfx.synth_i8(2, 1024, 64, seed=9, sigma=20.0, rho=0.3)).astype(np.float64)
+    z0 = r[:, 0, :, 0] + 1j * r[:, 0, :, 1]
+    z1 = r[:, 1, :, 0] + 1j * r[:, 1, :, 1]
+    rho = np.abs(np.mean(z0 * np.conj(z1))) / np.sqrt(np.mean(np.abs(z0) ** 2) * np.mean(np.abs(z1) ** 2))
+    assert abs(rho - 0.3) < 0.02
+
+
+@pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (256, 4096, 8, 64), (130, 512, 8, 100), (3, 64, 4, 9)])
+def test_fp16_split_vs_fp64(NN, N, T, Ns):
+    """FP16 hi+lo split X-engine within the 1e-5 autocorrelation-normalised bar
+    (operating point C3 shape at reduced integration length)."""
+    print(f"fp16split NN={NN} N={N} T={T} Ns={Ns}", flush=True)
+    plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=NN + N, sigma=20.0, rho=0.3, max_delay=8,
+                        tones=((0.05, 30.0),))
+    xin = plan.fengine(x8, plan.alloc_xinput())
+    v16 = _host(plan.xengine(xin, torch.full((N, plan.nb), complex(np.nan, np.nan), dtype=torch.complex64, device="cuda")))
+    assert np.all(np.isfinite(v16))
+    plan.set_precision(mdtfx.XPREC_FP32)
+    v32 = _host(plan.xengine(xin, plan.alloc_vis()))
+    W = _host(plan.window()).astype(np.float64)
+    chans = np.unique(np.linspace(0, N - 1, min(N, 24)).astype(int))
+    v64 = orc.fx_chain_f64(W, _host(x8), NN, N, T, Ns, chans)
+    e16 = _tri_autonorm_err(v16[chans], v64, NN)
+    e32 = _tri_autonorm_err(v32[chans], v64, NN)
+    print(f"max normalised error vs FP64: fp16-split {e16:.3e}  fp32 {e32:.3e}")
+    assert e16 < 1e-5 and e32 < 1e-5
