@@ -1,0 +1,213 @@
+"""FX correlator benchmark on one MI355X.
+
+Workload (operating point C3): 256 inputs, 8-tap PFB, 4096-point FFT, 2048
+spectra per dump, int8 samples sigma=20 with a common component (rho=0.3)
+delayed per input by 0-8 samples; FP16 hi+lo split X-engine with FP32
+accumulation and the FP64 cross-dump accumulator.  A step is one dump: the
+fused F-engine (PFB FIR + FFT + corner turn) and the X-engine over samples
+already resident in HBM.  The same dumps are timed with the FP32-MFMA
+X-engine for the metric's comparison.
+
+Prints ONE JSON line.  value = Msamples/s/input of the FP16-split path =
+Nspectra * Nchannels / (seconds per dump) / 1e6.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mdt_fpga-gpu_pipeline_amd"))
+
+METRIC = "FX Msamples/s/input at 256 inputs, FP16-split vs FP32 MFMA; max rel err vs FP64"
+NN, N, T, NS = 256, 4096, 8, 2048
+SEED, SIGMA, RHO, MAXDELAY = 2026, 20.0, 0.3, 8
+HBM_PEAK_GBS = 8000.0
+FP16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_path(torch, mdtfx, plan, x8, xin, vis, acc, steps, warmup):
+    """Run warmup + timed dumps; returns wall ms/dump and per-kernel mean ms
+    from HIP events recorded on the stream the kernels run on."""
+    for _ in range(warmup):
+        plan.dump(x8, xin, vis, acc)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record()
+        plan.fengine(x8, xin)
+        ev[k][1].record()
+        plan.xengine(xin, vis, acc)
+        ev[k][2].record()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    f_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    x_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    return wall_ms, f_ms, x_ms
+
+
+def load_traffic():
+    """HBM bytes per launch from the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    runs, scaled by the copy-kernel calibration (profiles/traffic_c3.json)."""
+    p = os.path.join(ROOT, "profiles", "traffic_c3.json")
+    if not os.path.exists(p):
+        return {}
+    with open(p) as f:
+        return json.load(f)
+
+
+def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, err_channels):
+    """The oracle's two legs on the host: (1) the FP32 CPU port of the chain
+    timed on a bounded sample (all inputs and channels, the first
+    sample_spectra spectra of the dump); (2) the FP64 oracle on err_channels
+    channels of the full dump, for the error of the GPU visibilities."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    W = plan.window().cpu().numpy()
+    x_host = x8.cpu().numpy()
+    nthreads = orc.num_threads()
+    sub = np.ascontiguousarray(x_host[: sample_spectra + T - 1])
+    orc.fx_chain_f32(W, np.ascontiguousarray(x_host[: 4 + T - 1]), NN, N, T, 4, nthreads)  # warm-up
+    t0 = time.perf_counter()
+    orc.fx_chain_f32(W, sub, NN, N, T, sample_spectra, nthreads)
+    dt = time.perf_counter() - t0
+    cpu = {
+        "value": sample_spectra * N / dt / 1e6,
+        "unit": "Msamples/s/input",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"FP32 CPU restatement of the chain (fmaf FIR, radix-2 FFT, corner turn, "
+                  f"FP32 triangle correlation) on all {NN} inputs x {N} channels, first "
+                  f"{sample_spectra} of {NS} spectra of the same dump; {dt:.1f} s",
+    }
+    chans = np.unique(np.linspace(0, N - 1, err_channels).astype(int))
+    t0 = time.perf_counter()
+    v64 = orc.fx_chain_f64(W.astype(np.float64), x_host, NN, N, T, NS, chans)
+    t64 = time.perf_counter() - t0
+    a, b = np.tril_indices(NN)
+    diag = np.array([orc.tri_index(i, i) for i in range(NN)])
+    auto = np.real(v64[:, diag])
+    norm = np.sqrt(auto[:, a] * auto[:, b])
+    big = np.abs(v64) > 1e-3 * norm
+
+    def errs(v):
+        d = np.abs(v[chans] - v64)
+        return {"autocorr_normalised_max": float(np.max(d / norm)),
+                "elementwise_rel_max_where_|V|>1e-3_norm": float(np.max(d[big] / np.abs(v64[big]))),
+                "per_channel_max": [float(x) for x in np.max(d / norm, axis=1)]}
+
+    err = {"fp16_split": errs(vis16), "fp32_mfma": errs(vis32), "channels_checked": [int(c) for c in chans],
+           "definition": "max over (channel, a>=b) of |V - V64| / sqrt(V64_aa V64_bb); FP64 oracle "
+                         "chain from the same int8 samples", "oracle_seconds": round(t64, 1)}
+    return cpu, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-spectra", type=int, default=256)
+    ap.add_argument("--err-channels", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 error legs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures a single MI355X; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is design-only here)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import mdtfx
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    log(f"bench: C3 NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
+    plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
+    plan.set_stream(stream)
+    x8 = mdtfx.synth_i8(NN, N, NS + T - 1, seed=SEED, sigma=SIGMA, rho=RHO, max_delay=MAXDELAY)
+    xin = plan.alloc_xinput()
+    vis16 = plan.alloc_vis()
+    vis32 = plan.alloc_vis()
+    acc = torch.zeros((N, plan.nb), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+
+    log("bench: FP16-split path")
+    w16, f16_ms, x16_ms = time_path(torch, mdtfx, plan, x8, xin, vis16, acc, args.steps, args.warmup)
+    plan.set_precision(mdtfx.XPREC_FP32)
+    log("bench: FP32-MFMA path")
+    w32, f32_ms, x32_ms = time_path(torch, mdtfx, plan, x8, xin, vis32, acc, args.steps, args.warmup)
+    plan.set_precision(mdtfx.XPREC_FP16_SPLIT)
+    torch.cuda.synchronize()
+
+    per_dump = NS * N
+    value16 = per_dump / (w16 * 1e-3) / 1e6
+    value32 = per_dump / (w32 * 1e-3) / 1e6
+    nb = NN * (NN + 1) // 2
+    f_bytes = NN * (NS + T - 1) * N * 2 + T * N * 4 + NN * NS * N * 8
+    x_flops = 8.0 * N * NS * nb
+    kernels = {
+        "fengine": {"ms": f16_ms, "algorithmic_bytes": f_bytes, "GB/s": f_bytes / (f16_ms * 1e-3) / 1e9},
+        "xengine_fp16split": {"ms": x16_ms, "useful_flops": x_flops,
+                              "useful_TFLOP/s": x_flops / (x16_ms * 1e-3) / 1e12,
+                              "issued_TFLOP/s": 3 * x_flops / (x16_ms * 1e-3) / 1e12},
+        "xengine_fp32": {"ms": x32_ms, "useful_flops": x_flops,
+                         "useful_TFLOP/s": x_flops / (x32_ms * 1e-3) / 1e12},
+        "fengine_in_fp32_path_ms": f32_ms,
+    }
+    traffic = load_traffic()
+    if f16_ms >= x16_ms:
+        ach = f_bytes / (f16_ms * 1e-3) / 1e9
+        roof = {"kernel": "fengine_kernel<4096,8>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get("fengine")}
+    else:
+        ach = x_flops / (x16_ms * 1e-3) / 1e12
+        roof = {"kernel": "xeng_f16split_kernel", "bound": "mfma", "achieved": ach, "peak": FP16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP16_PEAK_TFLOPS, "traffic": traffic.get("xengine_fp16split")}
+
+    out = {
+        "metric": METRIC,
+        "value": value16,
+        "unit": "Msamples/s/input",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": w16,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16-split",
+        "data": f"synthetic int8 (I,Q): sigma={SIGMA}, common component rho={RHO} delayed 0-{MAXDELAY} "
+                f"samples per input, seed {SEED}",
+        "config": {"workload": "C3 scaled array: 256 inputs, 8-tap PFB (BH4-windowed sinc), 4096-pt FFT, "
+                               "2048 spectra/dump, FP16 hi+lo split X-engine, FP32 accumulate, FP64 cross-dump "
+                               "accumulator",
+                   "inputs": NN, "taps": T, "channels": N, "spectra_per_dump": NS, "baselines": nb},
+        "fp32_mfma": {"value": value32, "ms_per_step": w32},
+        "speedup_fp16split_vs_fp32": value16 / value32,
+        "kernels": kernels,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline + FP64 error legs")
+        cpu, err = cpu_baseline_and_error(np, torch, plan, x8, vis16.cpu().numpy(), vis32.cpu().numpy(),
+                                          args.cpu_spectra, args.err_channels)
+        out["cpu_baseline"] = cpu
+        out["max_rel_err_vs_fp64"] = err
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
