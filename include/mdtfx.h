@@ -157,7 +157,8 @@ int mdt_unpack_u32(const uint32_t *words, float *out, size_t n, hipStream_t stre
 
 /* ---------------------------------------------------------------- native fused path
  * One dump: int8 (I,Q) samples [Ns+T-1][NN][N][2] -> fused PFB FIR + FFT +
- * corner turn -> channel-major X input [N][NN][Ns] complex FP32 -> MFMA
+ * corner turn -> channel-major X input [N][NN][Ns] complex FP32 (row and
+ * channel pitches below) -> MFMA
  * X-engine over the lower triangle -> visibilities [N][NN(NN+1)/2] complex
  * FP32, baseline bl(a,b) = a(a+1)/2 + b for a >= b, V = (1/Ns) sum_s
  * x_a conj(x_b) (the alpha of MDT_OneStation_client.cu:683).  Optionally adds
@@ -170,8 +171,19 @@ int mdt_fx_set_stream(mdt_fx_plan plan, hipStream_t stream);
 int mdt_fx_set_precision(mdt_fx_plan plan, int xprec);
 /* bytes of the channel-major X input buffer the caller must provide */
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes);
+/* X input layout: sample (channel k, input e, spectrum s) sits at element
+ * k*chan_pitch + e*row_pitch + s, row_pitch = Ns rounded up to even + row_pad,
+ * chan_pitch = NN*row_pitch + chan_pad.  The padding (even, in samples) keeps
+ * rows and channels off power-of-two strides; defaults 8 and 64. */
+int mdt_fx_set_xinput_padding(mdt_fx_plan plan, int row_pad, int chan_pad);
+int mdt_fx_xinput_pitch(mdt_fx_plan plan, size_t *row_pitch, size_t *chan_pitch);
 /* copies the plan's FP32 window [Ntaps][Nchannels] to a device buffer */
 int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
+/* The X input's sample format follows the plan's precision: FP32 (re, im) for
+ * MDT_XPREC_FP32; for MDT_XPREC_FP16_SPLIT each 8-byte sample holds FP16
+ * (hi_re, hi_im, lo_re, lo_im) of x*s, s the plan's power-of-two scale,
+ * hi = fp16(x*s), lo = fp16(x*s - hi).  Run the F-engine at the precision the
+ * X-engine will use. */
 int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
 int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_complex *Vis,
                    double *Acc64);

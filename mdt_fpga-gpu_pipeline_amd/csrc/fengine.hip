@@ -18,6 +18,12 @@
 
 namespace mdt {
 
+// persistent-kernel configuration (spectra per item, transforms per exchange,
+// corner-turn method)
+constexpr int FE_SB = 4;
+constexpr int FE_NX = 2;
+constexpr bool FE_DPPW = false;
+
 template <int N>
 struct FCfg {
   static constexpr int P = fft_points(N);
@@ -133,16 +139,19 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   if constexpr (SB == 1) {
     if (valid_e) {
 #pragma unroll
-      for (int m = 0; m < P; m++) a.out[((long)(t + m * TPB) * a.NN + e) * a.Ns + s0] = acc[0][m];
+      for (int m = 0; m < P; m++)
+        a.out[(long)(t + m * TPB) * a.cp + (long)e * a.rp + s0] =
+            a.split ? split_sample(acc[0][m], a.split_scale) : acc[0][m];
     }
   } else {
     constexpr int CH = SB / 2;  // 16-byte chunks per (k, e) row segment
-    const bool vec = ((a.Ns & 1) == 0);
+    const bool vec = ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
 #pragma unroll
     for (int m = 0; m < P; m++) {
       __syncthreads();
 #pragma unroll
-      for (int sl = 0; sl < SB; sl++) lds[t * C::ROW + sl] = acc[sl][m];
+      for (int sl = 0; sl < SB; sl++)
+        lds[t * C::ROW + sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
       __syncthreads();
       if (valid_e) {
 #pragma unroll
@@ -151,7 +160,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
           const int tt = q / CH, pr = q % CH;
           const int k = tt + m * TPB;
           const int sl = 2 * pr;
-          float2 *dst = a.out + ((long)k * a.NN + e) * a.Ns + s0 + sl;
+          float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0 + sl;
           const float2 v0 = lds[tt * C::ROW + sl], v1 = lds[tt * C::ROW + sl + 1];
           if (vec && sl + 1 < scount) {
             *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
@@ -165,9 +174,206 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   }
 }
 
+// Persistent variant for N in [512, 4096] (8 points per thread) and T in
+// {4, 8}: one workgroup per slot of the CU, looping over (input, spectrum
+// block) items.  Each thread keeps its T x 8 window taps in registers for the
+// whole kernel (the window is read from memory once per workgroup, not once per
+// spectrum), and the workgroup stages the per-pass twiddle table in LDS once.
+// Same arithmetic in the same order as fengine_kernel, so the output is
+// bit-identical.
+__device__ __forceinline__ float swap_pair(float v) {  // lane l <-> l ^ 1 (DPP quad_perm 1,0,3,2)
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+// NX: transforms per LDS exchange round.  DPPW: corner turn through lane-pair
+// DPP swaps (SB = 4) instead of LDS staging.
+template <int N, int T, int SB, int NX, bool DPPW>
+__global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, long nitems) {
+  static_assert(!DPPW || SB == 4, "the lane-pair corner turn writes 4 spectra per segment");
+  static_assert(SB % NX == 0, "spectra per block must be a multiple of NX");
+  constexpr int P = 8, TPB = N / P;
+  constexpr int L = full_passes(N, P);
+  constexpr int RL = N / ipow_c(P, L);
+  constexpr int NPASS = L + (RL > 1 ? 1 : 0);
+  constexpr int TWN = pass_tw_entries<N, P>();
+  constexpr int ROW = SB + 1;
+  constexpr int FBUF = NX * lds_entries<N>();
+  constexpr int BUF = (DPPW || FBUF >= TPB * ROW) ? FBUF : TPB * ROW;
+  __shared__ float2 twl[TWN > 0 ? TWN : 1];
+  __shared__ float2 buf[BUF];
+  const int t = threadIdx.x;
+
+  // per-pass twiddle table: entry [pass][jm][r] = W_{NS*R}^{jm*r}
+  for (int idx = t; idx < TWN; idx += TPB) {
+    int pass = 1, off = 0, ns = P;
+    while (pass < NPASS - 1 && idx >= off + ns * 8) {
+      off += ns * 8;
+      ns *= P;
+      pass++;
+    }
+    const int R = (pass < L) ? P : RL;
+    const int rel = idx - off, jm = rel / 8, r = rel % 8;
+    twl[idx] = (r < R) ? a.tw[jm * r * (N / (ns * R))] : make_float2(0.f, 0.f);
+  }
+  float w[T][P];
+#pragma unroll
+  for (int i = 0; i < T; i++)
+#pragma unroll
+    for (int m = 0; m < P; m++) w[i][m] = a.W[(long)i * N + t + m * TPB];
+  __syncthreads();
+  const PassTw<P> tws{twl};
+
+  const int nsb = (a.Ns + SB - 1) / SB;
+  const long G = gridDim.x;
+  const long b = blockIdx.x;
+  const long spec = (long)a.NN * N;
+  for (long base = 0; base < nitems; base += G) {
+    // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
+    // spectrum blocks of one input) go to the same XCD in each round.
+    const long logical = (G % 8 == 0) ? base + (b % 8) * (G / 8) + b / 8 : base + b;
+    if (logical >= nitems) break;
+    const int sb = (int)(logical % nsb);
+    const int e = (int)(logical / nsb);
+    const int s0 = sb * SB;
+    const int scount = min(SB, a.Ns - s0);
+    const char2 *src = a.in + (long)s0 * spec + (long)e * N + t;
+
+    float2 acc[SB][P];
+#pragma unroll
+    for (int sl = 0; sl < SB; sl++)
+#pragma unroll
+      for (int m = 0; m < P; m++) acc[sl][m] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int f = 0; f < SB + T - 1; f++) {
+      if (f < scount + T - 1) {
+        float2 x[P];
+#pragma unroll
+        for (int m = 0; m < P; m++) {
+          char2 v = src[(long)f * spec + m * TPB];
+          x[m] = make_float2((float)v.x, (float)v.y);
+        }
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) {
+          const int i = f - sl;
+          if (i >= 0 && i < T && sl < scount) {
+#pragma unroll
+            for (int m = 0; m < P; m++) {
+              acc[sl][m].x = __builtin_fmaf(w[i][m], x[m].x, acc[sl][m].x);
+              acc[sl][m].y = __builtin_fmaf(w[i][m], x[m].y, acc[sl][m].y);
+            }
+          }
+        }
+      }
+    }
+    // transforms in pairs (block-uniform trip count)
+#pragma unroll
+    for (int sp = 0; sp < SB; sp += NX) {
+      if (sp < scount) {
+        float2 (&pair)[NX][P] = *reinterpret_cast<float2(*)[NX][P]>(&acc[sp][0]);
+        fft_block_multi<N, P, false, NX>(pair, t, buf, tws);
+      }
+    }
+
+    if (a.gain != nullptr) {
+      const float2 g = a.gain[e];
+      const float tau = a.delay ? a.delay[e] : 0.0f;
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        const int k = t + m * TPB;
+        const int kk = (k < N / 2) ? k : k - N;
+        float sn, cs;
+        sincospif(-2.0f * (float)kk * tau / (float)N, &sn, &cs);
+        const float2 ph = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) acc[sl][m] = cmul(acc[sl][m], ph);
+      }
+    }
+
+    if constexpr (DPPW) {
+      // corner turn without LDS: lanes 2g and 2g+1 hold channels k and k+1 for
+      // 4 spectra (two 16-byte chunks each); one DPP swap gives each lane pair
+      // the two halves of one channel's 32-byte segment for each of 2 stores.
+      const int odd = t & 1;
+      const bool full = (scount == SB) && ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        const int k = t + m * TPB;
+        float2 o[SB];
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) o[sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
+        if (full) {
+          const float4 c0 = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
+          const float4 c1 = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
+          const float4 snd = odd ? c0 : c1;
+          const float4 rcv =
+              make_float4(swap_pair(snd.x), swap_pair(snd.y), swap_pair(snd.z), swap_pair(snd.w));
+          float4 *d1 = (float4 *)(a.out + (long)(k - odd) * a.cp + (long)e * a.rp + s0 + 2 * odd);
+          float4 *d2 = (float4 *)(a.out + (long)(k + 1 - odd) * a.cp + (long)e * a.rp + s0 + 2 * odd);
+          *d1 = odd ? rcv : c0;
+          *d2 = odd ? c1 : rcv;
+        } else {
+          float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0;
+#pragma unroll
+          for (int sl = 0; sl < SB; sl++)
+            if (sl < scount) dst[sl] = o[sl];
+        }
+      }
+    } else {
+      // corner turn staged per m through LDS: each store instruction writes
+      // SB consecutive spectra of a (channel, input) row per SB/2 lanes
+      constexpr int CH = SB / 2;
+      const bool vec = ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        __syncthreads();
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++)
+          buf[t * ROW + sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < CH; v++) {
+          const int q = t + v * TPB;
+          const int tt = q / CH, pr = q % CH;
+          const int k = tt + m * TPB;
+          const int sl = 2 * pr;
+          float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0 + sl;
+          const float2 v0 = buf[tt * ROW + sl], v1 = buf[tt * ROW + sl + 1];
+          if (vec && sl + 1 < scount) {
+            *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+          } else {
+            if (sl < scount) dst[0] = v0;
+            if (sl + 1 < scount) dst[1] = v1;
+          }
+        }
+      }
+      __syncthreads();  // the staging buffer is the next item's FFT buffer
+    }
+  }
+}
+
+template <int N, int T, int SB, int NX, bool DPPW>
+static int launch_persistent(const FArgs &a, hipStream_t st) {
+  const long nitems = (long)a.NN * ((a.Ns + SB - 1) / SB);
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SB, NX, DPPW>, N / 8,
+                                                   0) != hipSuccess)
+    return fail(MDT_ERR_DEVICE, "device attributes for the F-engine grid");
+  long grid = (long)cus * (per_cu > 0 ? per_cu : 1);
+  if (grid > nitems) grid = nitems;
+  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SB, NX, DPPW>), dim3((unsigned)grid), dim3(N / 8), 0, st, a,
+                     nitems);
+  return check_launch("fengine_persistent_kernel");
+}
+
 template <int N>
 static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   using C = FCfg<N>;
+  if constexpr (N >= 512 && N <= 4096) {
+    if (a.T == 4) return launch_persistent<N, 4, FE_SB, FE_NX, FE_DPPW>(a, st);
+    if (a.T == 8) return launch_persistent<N, 8, FE_SB, FE_NX, FE_DPPW>(a, st);
+  }
   const long nsb = (a.Ns + C::SB - 1) / C::SB;
   const long neg = (a.NN + C::EB - 1) / C::EB;
   const long total = nsb * neg;

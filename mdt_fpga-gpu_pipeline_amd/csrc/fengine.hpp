@@ -9,11 +9,27 @@ struct FArgs {
   const char2 *in;      // [Ns+T-1][NN][N] int8 (I, Q)
   const float *W;       // [T][N] FP32 window
   const float2 *tw;     // W_N^k twiddles
-  float2 *out;          // [N][NN][Ns] complex FP32
+  float2 *out;          // channel-major: (k, e, s) at k*cp + e*rp + s, complex FP32
   int NN, Ns, T;
+  long rp, cp;          // row pitch (>= Ns) and channel pitch (>= NN*rp), in samples
   const float2 *gain;   // optional [NN] complex gains
   const float *delay;   // optional [NN] delays in samples
+  int split;            // 0: FP32 (re, im); 1: FP16 (hi_re, hi_im, lo_re, lo_im) of x*split_scale
+  float split_scale;    // exact power of two
 };
+
+// FP16 hi/lo split of one complex sample scaled by an exact power of two:
+// hi = fp16(x*sc), lo = fp16(x*sc - hi); 8 bytes (hi_re, hi_im, lo_re, lo_im).
+__device__ __forceinline__ float2 split_sample(float2 v, float sc) {
+  const float xr = v.x * sc, xi = v.y * sc;
+  const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+  const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+  const unsigned w0 = (unsigned)__builtin_bit_cast(unsigned short, hr) |
+                      ((unsigned)__builtin_bit_cast(unsigned short, hi_) << 16);
+  const unsigned w1 = (unsigned)__builtin_bit_cast(unsigned short, lr) |
+                      ((unsigned)__builtin_bit_cast(unsigned short, li) << 16);
+  return make_float2(__uint_as_float(w0), __uint_as_float(w1));
+}
 
 int launch_fengine(const FArgs &a, int N, hipStream_t st);
 

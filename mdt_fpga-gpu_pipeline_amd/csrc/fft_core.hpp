@@ -19,6 +19,7 @@
 namespace mdt {
 
 constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v / 2); }
 constexpr int full_passes(int n, int p) {
   int l = 0;
   while (n > 1 && n % p == 0) {
@@ -51,9 +52,9 @@ __device__ __forceinline__ float2 rot(float2 z) {
   } else {
     const float c = kCos16[idx], s = kSin16[idx];
     if constexpr (INV)
-      return make_float2(z.x * c - z.y * s, z.y * c + z.x * s);
+      return make_float2(__builtin_fmaf(z.x, c, -(z.y * s)), __builtin_fmaf(z.y, c, z.x * s));
     else
-      return make_float2(z.x * c + z.y * s, z.y * c - z.x * s);
+      return make_float2(__builtin_fmaf(z.x, c, z.y * s), __builtin_fmaf(z.y, c, -(z.x * s)));
   }
 }
 
@@ -102,9 +103,44 @@ constexpr int lds_entries() {
   return N + N / 16;
 }
 
-template <int N, int P, int R, int NS, bool LAST, bool INV>
-__device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds,
-                                         const float2 *__restrict__ tw) {
+// Twiddle sources.  GlobalTw: the W_N^k table (global memory, L1/L2 cached).
+// PassTw: per-pass table in LDS, entry [pass][j mod NS][r] holding
+// W_{NS*R}^{(j mod NS)*r}, so one butterfly reads its R twiddles as R/2
+// contiguous 16-byte words (radix-8 passes only).
+struct GlobalTw {
+  const float2 *__restrict__ tw;
+  template <int N, int NS, int R>
+  __device__ __forceinline__ float2 get(int jm, int r) const {
+    return tw[jm * r * (N / (NS * R))];
+  }
+};
+template <int P>
+constexpr int pass_tw_offset(int pass) {  // entries before pass `pass` (pass >= 1)
+  int off = 0, ns = P;
+  for (int p = 1; p < pass; p++) {
+    off += ns * 8;
+    ns *= P;
+  }
+  return off;
+}
+template <int N, int P>
+constexpr int pass_tw_entries() {
+  constexpr int L = full_passes(N, P);
+  constexpr int NPASS = L + ((N / ipow_c(P, L)) > 1 ? 1 : 0);
+  return pass_tw_offset<P>(NPASS);
+}
+template <int P>
+struct PassTw {
+  const float2 *tab;  // LDS
+  template <int N, int NS, int R>
+  __device__ __forceinline__ float2 get(int jm, int r) const {
+    constexpr int pass = ilog2_c(NS) / ilog2_c(P);
+    return tab[pass_tw_offset<P>(pass) + jm * 8 + r];
+  }
+};
+
+template <int N, int P, int R, int NS, bool LAST, bool INV, typename TW>
+__device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
   constexpr int TPB = N / P;
   constexpr int Q = P / R;
 #pragma unroll
@@ -115,10 +151,9 @@ __device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds,
     for (int r = 0; r < R; r++) v[r] = x[q + r * Q];
     const int jm = j & (NS - 1);
     if constexpr (NS > 1) {
-      constexpr int STEP = N / (NS * R);
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        float2 w = tw[jm * r * STEP];
+        float2 w = tw.template get<N, NS, R>(jm, r);
         if constexpr (INV) w.y = -w.y;
         v[r] = cmul(v[r], w);
       }
@@ -135,9 +170,8 @@ __device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds,
   }
 }
 
-template <int N, int P, bool INV, int PASS>
-__device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds,
-                                          const float2 *__restrict__ tw) {
+template <int N, int P, bool INV, int PASS, typename TW>
+__device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
   constexpr int L = full_passes(N, P);
   constexpr int RL = N / ipow_c(P, L);
   constexpr int NPASS = L + (RL > 1 ? 1 : 0);
@@ -154,15 +188,49 @@ __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds,
   if constexpr (PASS + 1 < NPASS) fft_stage<N, P, INV, PASS + 1>(x, t, lds, tw);
 }
 
+// NX independent transforms advanced together: each pass does the NX sets of
+// butterflies, one pair of barriers per exchange round for all of them.
+template <int N, int P, bool INV, int PASS, int NX, typename TW>
+__device__ __forceinline__ void fft_stage_multi(float2 (&x)[NX][P], int t, float2 *lds,
+                                                const TW &tw) {
+  constexpr int L = full_passes(N, P);
+  constexpr int RL = N / ipow_c(P, L);
+  constexpr int NPASS = L + (RL > 1 ? 1 : 0);
+  constexpr int R = (PASS < L) ? P : RL;
+  constexpr int NS = ipow_c(P, PASS);
+  constexpr int TPB = N / P;
+  if constexpr (PASS > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NX; v++)
+#pragma unroll
+      for (int m = 0; m < P; m++) x[v][m] = lds[v * lds_entries<N>() + lpad(t + m * TPB)];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < NX; v++)
+    fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x[v], t, lds + v * lds_entries<N>(), tw);
+  if constexpr (PASS + 1 < NPASS) fft_stage_multi<N, P, INV, PASS + 1, NX>(x, t, lds, tw);
+}
+template <int N, int P, bool INV, int NX, typename TW>
+__device__ __forceinline__ void fft_block_multi(float2 (&x)[NX][P], int t, float2 *lds,
+                                                const TW &tw) {
+  fft_stage_multi<N, P, INV, 0, NX>(x, t, lds, tw);
+}
+
 // Transform one N-point vector held as x[m] = element t + m*(N/P).  All
 // threads of the block must call this the same number of times (it contains
 // __syncthreads when N > P).  lds: lds_entries<N>() float2 private to this
 // transform.
+template <int N, int P, bool INV, typename TW>
+__device__ __forceinline__ void fft_block_tw(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
+  static_assert(N % P == 0 && N >= P, "N must be a multiple of P");
+  fft_stage<N, P, INV, 0>(x, t, lds, tw);
+}
 template <int N, int P, bool INV>
 __device__ __forceinline__ void fft_block(float2 (&x)[P], int t, float2 *lds,
                                           const float2 *__restrict__ tw) {
-  static_assert(N % P == 0 && N >= P, "N must be a multiple of P");
-  fft_stage<N, P, INV, 0>(x, t, lds, tw);
+  fft_block_tw<N, P, INV>(x, t, lds, GlobalTw{tw});
 }
 
 // Twiddle table W_N^k = exp(-2 pi i k / N), k < N, FP64 evaluated, FP32 stored.

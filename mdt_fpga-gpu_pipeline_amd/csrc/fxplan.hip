@@ -21,6 +21,9 @@ struct mdt_fx_plan_s {
   const float2 *gain = nullptr;
   const float *delay = nullptr;
   float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15
+  int row_pad = 8, chan_pad = 64;  // X input padding (samples) after each row / channel
+  long rp() const { return (long)Ns + (Ns & 1) + row_pad; }  // even: 16-byte rows
+  long cp() const { return (long)NN * rp() + chan_pad; }
 };
 
 using namespace mdt;
@@ -105,7 +108,26 @@ int mdt_fx_set_precision(mdt_fx_plan plan, int xprec) {
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(bytes);
-  *bytes = sizeof(float2) * (size_t)plan->N * plan->NN * plan->Ns;
+  *bytes = sizeof(float2) * (size_t)plan->N * (size_t)plan->cp();
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_set_xinput_padding(mdt_fx_plan plan, int row_pad, int chan_pad) {
+  MDT_REQUIRE_PTR(plan);
+  if (row_pad < 0 || chan_pad < 0 || (row_pad & 1) || (chan_pad & 1) || row_pad > 4096 ||
+      chan_pad > (1 << 20))
+    return fail(MDT_ERR_INVALID_VALUE, "padding must be even and in [0, 4096] / [0, 2^20]");
+  plan->row_pad = row_pad;
+  plan->chan_pad = chan_pad;
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_xinput_pitch(mdt_fx_plan plan, size_t *row_pitch, size_t *chan_pitch) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(row_pitch);
+  MDT_REQUIRE_PTR(chan_pitch);
+  *row_pitch = (size_t)plan->rp();
+  *chan_pitch = (size_t)plan->cp();
   return MDT_SUCCESS;
 }
 
@@ -131,8 +153,12 @@ int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInput
   a.NN = plan->NN;
   a.Ns = plan->Ns;
   a.T = plan->T;
+  a.rp = plan->rp();
+  a.cp = plan->cp();
   a.gain = plan->gain;
   a.delay = plan->delay;
+  a.split = (plan->xprec == MDT_XPREC_FP16_SPLIT) ? 1 : 0;
+  a.split_scale = plan->f16_scale;
   return launch_fengine(a, plan->N, plan->stream);
 }
 
@@ -149,8 +175,8 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
   g.alpha = make_float2(1.0f / (float)plan->Ns, 0.f);  // MDT_OneStation_client.cu:683
   g.beta = make_float2(0.f, 0.f);
   g.Abase = g.Bbase = (const float2 *)XInputNative;
-  g.Abatch = g.Bbatch = (long)NN * plan->Ns;
-  g.ldI = g.ldJ = plan->Ns;
+  g.Abatch = g.Bbatch = plan->cp();
+  g.ldI = g.ldJ = plan->rp();
   g.Cbase = (float2 *)Vis;
   g.Cbatch = (long)NN * (NN + 1) / 2;
   g.herm = 1;

@@ -33,51 +33,53 @@ constexpr int XPLANE = 2 * XRL;  // floats per sample: re row + im row
 
 // Load a 64-row x KC-sample operand chunk into LDS [s][comp][row].
 // KM: element (row, s) at base[row + s*ld] (column-major, rows contiguous).
+// MK: element (row, s) at base[row*ld + s] (samples contiguous, 16-byte
+//     aligned rows: even ld).
+// Branch-free: out-of-range rows/samples read a clamped valid address and are
+// zeroed, so all loads of a chunk are in flight together.
 template <int LAYOUT>
 __device__ __forceinline__ void load_chunk(float *lds, const float2 *__restrict__ base, long ld,
                                            int row0, int nrows, int s0, int K) {
   const int tid = threadIdx.x;
   if constexpr (LAYOUT == XL_KM) {
     const int r = tid & 63, sq = tid >> 6;  // 64 rows x 4 sample lanes
+    const bool r_ok = row0 + r < nrows;
+    const long rr = r_ok ? row0 + r : 0;
+    float2 v[XKC / 4];
+#pragma unroll
+    for (int q = 0; q < XKC / 4; q++) {
+      const int s = s0 + sq + 4 * q;
+      const bool ok = r_ok && s < K;
+      const float2 x = base[rr + (long)(s < K ? s : 0) * ld];
+      v[q] = ok ? x : make_float2(0.f, 0.f);
+    }
 #pragma unroll
     for (int q = 0; q < XKC / 4; q++) {
       const int s = sq + 4 * q;
-      float2 v = make_float2(0.f, 0.f);
-      if (row0 + r < nrows && s0 + s < K) v = base[(long)(row0 + r) + (long)(s0 + s) * ld];
-      lds[s * XPLANE + r] = v.x;
-      lds[s * XPLANE + XRL + r] = v.y;
+      lds[s * XPLANE + r] = v[q].x;
+      lds[s * XPLANE + XRL + r] = v[q].y;
     }
   } else {
-    // MK: element (row, s) at base[row*ld + s] (samples contiguous); 16 lanes
-    // x 16 bytes cover 32 samples of one row.
     const int sp = tid & 15, rq = tid >> 4;  // 16 sample pairs x 16 rows
+    const int s = 2 * sp;
+    const bool s_ok0 = s0 + s < K, s_ok1 = s0 + s + 1 < K;
+    const int sc = s_ok0 ? s0 + s : 0;
+    float4 v[XT / 16];
+#pragma unroll
+    for (int q = 0; q < XT / 16; q++) {
+      const int grow = row0 + rq + 16 * q;
+      const bool r_ok = grow < nrows;
+      const float4 x = *(const float4 *)(base + (long)(r_ok ? grow : 0) * ld + sc);
+      const bool ok0 = r_ok && s_ok0, ok1 = r_ok && s_ok1;
+      v[q] = make_float4(ok0 ? x.x : 0.f, ok0 ? x.y : 0.f, ok1 ? x.z : 0.f, ok1 ? x.w : 0.f);
+    }
 #pragma unroll
     for (int q = 0; q < XT / 16; q++) {
       const int r = rq + 16 * q;
-      const int s = 2 * sp;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      const long grow = row0 + r;
-      if (grow < nrows) {
-        const float2 *p = base + grow * ld + s0 + s;
-        if (s0 + s + 1 < K && (((uintptr_t)p & 15) == 0)) {
-          v = *(const float4 *)p;
-        } else {
-          if (s0 + s < K) {
-            float2 a = p[0];
-            v.x = a.x;
-            v.y = a.y;
-          }
-          if (s0 + s + 1 < K) {
-            float2 b = p[1];
-            v.z = b.x;
-            v.w = b.y;
-          }
-        }
-      }
-      lds[s * XPLANE + r] = v.x;
-      lds[s * XPLANE + XRL + r] = v.y;
-      lds[(s + 1) * XPLANE + r] = v.z;
-      lds[(s + 1) * XPLANE + XRL + r] = v.w;
+      lds[s * XPLANE + r] = v[q].x;
+      lds[s * XPLANE + XRL + r] = v[q].y;
+      lds[(s + 1) * XPLANE + r] = v[q].z;
+      lds[(s + 1) * XPLANE + XRL + r] = v[q].w;
     }
   }
 }

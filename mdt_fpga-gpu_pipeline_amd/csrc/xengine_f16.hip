@@ -1,6 +1,7 @@
 // xengine_f16.hip -- the split-precision X-engine of the native path on
 // v_mfma_f32_32x32x16_f16: each FP32 sample x (pre-scaled by an exact power
-// of two) is split into hi = fp16(x) and lo = fp16(x - hi), and
+// of two) is split into hi = fp16(x) and lo = fp16(x - hi) by the F-engine's
+// epilogue, and
 //   x_i * x_j ~= hi_i hi_j + hi_i lo_j + lo_i hi_j        (FP32 accumulate)
 // which keeps ~22 significant bits per operand; the dropped lo_i lo_j term is
 // below 2^-22 relative.  Same complex scheme as the FP32 kernel: K runs over
@@ -9,8 +10,7 @@
 //
 // Workgroup tile 128 x 128 complex (4 waves of 64 x 64 = 2 x 2 MFMA tiles),
 // K streamed in chunks of 16 samples; the next chunk is prefetched into
-// registers while the current one is multiplied out of LDS.  FP32 -> hi/lo
-// conversion happens once per element on the way into LDS.  Only tiles on or
+// registers while the current one is multiplied out of LDS.  Only tiles on or
 // below the diagonal are visited and 32 x 32 MFMA tiles strictly above it are
 // skipped.  Output: packed lower triangle, optional FP64 accumulation.
 #include "common.hpp"
@@ -32,18 +32,6 @@ __device__ __forceinline__ unsigned pack_h2(_Float16 a, _Float16 b) {
          ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
 }
 
-// hi/lo split of two complex samples (4 floats) scaled by an exact power of 2
-__device__ __forceinline__ void split4(float4 v, float sc, uint2 &hi, uint2 &lo) {
-  float x[4] = {v.x * sc, v.y * sc, v.z * sc, v.w * sc};
-  _Float16 h[4], l[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    h[q] = (_Float16)x[q];
-    l[q] = (_Float16)(x[q] - (float)h[q]);
-  }
-  hi = make_uint2(pack_h2(h[0], h[1]), pack_h2(h[2], h[3]));
-  lo = make_uint2(pack_h2(l[0], l[1]), pack_h2(l[2], l[3]));
-}
 
 // (re, im) half pairs -> (-im, re)
 __device__ __forceinline__ half8 rot_conj(half8 v) {
@@ -55,55 +43,52 @@ __device__ __forceinline__ half8 rot_conj(half8 v) {
 
 struct Prefetch {
   float4 v[4];
+  unsigned ok;  // bit 2q: first sample of row q valid, bit 2q+1: second
 };
 
-// global -> registers: 128 rows x 16 samples (8 lanes x 16 B per row)
+// global -> registers: 128 rows x 16 samples (8 lanes x 16 B per row).
+// Branch-free so the four loads are in flight together: out-of-range rows and
+// samples read a clamped, valid address and are zeroed afterwards.  Rows are
+// 16-byte aligned (even row pitch, even channel pitch).
 __device__ __forceinline__ void fetch(Prefetch &p, const float2 *__restrict__ base, long ld,
                                       int row0, int nrows, int s0, int K) {
   const int seg = threadIdx.x & 7, rq = threadIdx.x >> 3;
+  const int s = s0 + 2 * seg;
+  const bool s_ok0 = s < K, s_ok1 = s + 1 < K;
+  const int sc = s_ok0 ? s : 0;
+  unsigned ok = 0;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int row = rq + 32 * q;
-    const long grow = row0 + row;
-    const int s = s0 + 2 * seg;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (grow < nrows) {
-      const float2 *src = base + grow * ld + s;
-      if (s + 1 < K && (((uintptr_t)src & 15) == 0)) {
-        v = *(const float4 *)src;
-      } else {
-        if (s < K) {
-          float2 a = src[0];
-          v.x = a.x;
-          v.y = a.y;
-        }
-        if (s + 1 < K) {
-          float2 b = src[1];
-          v.z = b.x;
-          v.w = b.y;
-        }
-      }
-    }
-    p.v[q] = v;
+    const int row = row0 + rq + 32 * q;
+    const bool r_ok = row < nrows;
+    p.v[q] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + sc);
+    ok |= ((r_ok && s_ok0) ? 1u : 0u) << (2 * q);
+    ok |= ((r_ok && s_ok1) ? 1u : 0u) << (2 * q + 1);
   }
+  p.ok = ok;  // masks applied when staging, so the loads stay in flight
 }
 
-// registers -> LDS hi/lo planes, [row][k] with k = 2*s + (re|im)
-__device__ __forceinline__ void stage(const Prefetch &p, char *lds, float sc) {
+// registers -> LDS hi/lo planes, [row][k] with k = 2*s + (re|im).  The X
+// input already holds each sample as (hi_re, hi_im | lo_re, lo_im) halves (the
+// F-engine's split epilogue), so staging only routes dwords to the planes.
+__device__ __forceinline__ void stage(const Prefetch &p, char *lds) {
   const int seg = threadIdx.x & 7, rq = threadIdx.x >> 3;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int row = rq + 32 * q;
-    uint2 hi, lo;
-    split4(p.v[q], sc, hi, lo);
-    *(uint2 *)(lds + row * HROWB + seg * 8) = hi;
-    *(uint2 *)(lds + HPLANE + row * HROWB + seg * 8) = lo;
+    const float4 v = p.v[q];
+    const unsigned m0 = (p.ok >> (2 * q)) & 1u ? 0xFFFFFFFFu : 0u;
+    const unsigned m1 = (p.ok >> (2 * q + 1)) & 1u ? 0xFFFFFFFFu : 0u;
+    *(uint2 *)(lds + row * HROWB + seg * 8) =
+        make_uint2(__float_as_uint(v.x) & m0, __float_as_uint(v.z) & m1);
+    *(uint2 *)(lds + HPLANE + row * HROWB + seg * 8) =
+        make_uint2(__float_as_uint(v.y) & m0, __float_as_uint(v.w) & m1);
   }
 }
 
 __device__ __forceinline__ long tri_bl16(int a, int b) { return (long)a * (a + 1) / 2 + b; }
 
-__global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float sc, float oscale) {
+__global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float oscale) {
   __shared__ __attribute__((aligned(16))) char lds[2 * 2 * HPLANE];  // [op I|J][hi|lo][row][k]
 
   const long nwg = gridDim.x;
@@ -153,8 +138,8 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float sc
 
   for (int s0 = 0; s0 < K; s0 += HKC) {
     __syncthreads();
-    stage(pI, lds, sc);
-    if (!same) stage(pJ, lds + 2 * HPLANE, sc);
+    stage(pI, lds);
+    if (!same) stage(pJ, lds + 2 * HPLANE);
     __syncthreads();
     if (s0 + HKC < K) {  // prefetch the next chunk while this one is multiplied
       fetch(pI, base, g.ldI, bi * HT, NN, s0 + HKC, K);
@@ -179,18 +164,22 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float sc
           Jh[u] = *(const half8 *)(ldsJ + row * HROWB + koff);
           Jl[u] = *(const half8 *)(ldsJ + HPLANE + row * HROWB + koff);
         }
+        // three split products, each swept over the 2 x 2 tiles so that
+        // dependent MFMAs on one accumulator are 8 instructions apart
 #pragma unroll
-        for (int t = 0; t < 2; t++)
+        for (int p = 0; p < 3; p++) {
 #pragma unroll
-          for (int u = 0; u < 2; u++) {
-            if (!act[t][u]) continue;
-            accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih[t], Jh[u], accR[t][u], 0, 0, 0);
-            accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph[t], Jh[u], accI[t][u], 0, 0, 0);
-            accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih[t], Jl[u], accR[t][u], 0, 0, 0);
-            accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph[t], Jl[u], accI[t][u], 0, 0, 0);
-            accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il[t], Jh[u], accR[t][u], 0, 0, 0);
-            accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl[t], Jh[u], accI[t][u], 0, 0, 0);
-          }
+          for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+              if (!act[t][u]) continue;
+              const half8 a = (p == 2) ? Il[t] : Ih[t];
+              const half8 ar = (p == 2) ? Pl[t] : Ph[t];
+              const half8 bb = (p == 1) ? Jl[u] : Jh[u];
+              accR[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb, accR[t][u], 0, 0, 0);
+              accI[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ar, bb, accI[t][u], 0, 0, 0);
+            }
+        }
       }
     }
   }
@@ -232,7 +221,7 @@ int xeng_launch_f16split(const XArgs &g0, hipStream_t st) {
   const float sc = g.in_scale;
   if (!(sc > 0.f)) return fail(MDT_ERR_INVALID_VALUE, "FP16 split needs a positive input scale");
   const float oscale = g.alpha.x / (sc * sc);
-  hipLaunchKernelGGL(xeng_f16split_kernel, dim3((unsigned)total), dim3(256), 0, st, g, sc, oscale);
+  hipLaunchKernelGGL(xeng_f16split_kernel, dim3((unsigned)total), dim3(256), 0, st, g, oscale);
   return check_launch("xeng_f16split_kernel");
 }
 

@@ -87,6 +87,8 @@ _SIGS = {
     "mdt_fx_set_stream": [_VP, _VP],
     "mdt_fx_set_precision": [_VP, _I],
     "mdt_fx_xinput_bytes": [_VP, ctypes.POINTER(_SZ)],
+    "mdt_fx_set_xinput_padding": [_VP, _I, _I],
+    "mdt_fx_xinput_pitch": [_VP, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)],
     "mdt_fx_get_window": [_VP, _VP],
     "mdt_fx_fengine": [_VP, _VP, _VP],
     "mdt_fx_xengine": [_VP, _VP, _VP, _VP],
@@ -280,22 +282,55 @@ class FxPlan:
         check(lib().mdt_fx_get_window(self._h, _ptr(W, torch.float32)))
         return W
 
+    def set_xinput_padding(self, row_pad, chan_pad):
+        check(lib().mdt_fx_set_xinput_padding(self._h, row_pad, chan_pad))
+
+    def xinput_pitch(self):
+        rp, cp = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().mdt_fx_xinput_pitch(self._h, ctypes.byref(rp), ctypes.byref(cp)))
+        return rp.value, cp.value
+
     def alloc_xinput(self):
-        return torch.empty((self.N, self.NN, self.Ns), dtype=torch.complex64, device="cuda")
+        """X input buffer; returns the padded storage (contiguous, what the
+        library is given).  Use xinput_view() for the [N][NN][Ns] view."""
+        rp, cp = self.xinput_pitch()
+        return torch.empty((self.N, cp), dtype=torch.complex64, device="cuda")
+
+    def xinput_view(self, buf):
+        rp, cp = self.xinput_pitch()
+        return torch.as_strided(buf, (self.N, self.NN, self.Ns), (cp, rp, 1))
 
     def alloc_vis(self):
         return torch.empty((self.N, self.nb), dtype=torch.complex64, device="cuda")
 
+    def _check_sizes(self, x8=None, xin=None, vis=None, acc64=None):
+        """The kernels take raw pointers: refuse buffers smaller than the
+        extents they read or write, before anything is launched."""
+        if x8 is not None and x8.numel() < (self.Ns + self.T - 1) * self.NN * self.N * 2:
+            raise ValueError("input buffer smaller than (Ns+T-1) x NN x N int8 (I, Q) samples")
+        if xin is not None:
+            nbytes = ctypes.c_size_t()
+            check(lib().mdt_fx_xinput_bytes(self._h, ctypes.byref(nbytes)))
+            if xin.numel() * xin.element_size() < nbytes.value:
+                raise ValueError(f"X input buffer smaller than mdt_fx_xinput_bytes = {nbytes.value}")
+        if vis is not None and vis.numel() < self.N * self.nb:
+            raise ValueError("visibility buffer smaller than N x NN(NN+1)/2")
+        if acc64 is not None and acc64.numel() < self.N * self.nb:
+            raise ValueError("accumulator smaller than N x NN(NN+1)/2")
+
     def fengine(self, x8, xin):
+        self._check_sizes(x8=x8, xin=xin)
         check(lib().mdt_fx_fengine(self._h, _ptr(x8, torch.int8), _ptr(xin, torch.complex64)))
         return xin
 
     def xengine(self, xin, vis, acc64=None):
+        self._check_sizes(xin=xin, vis=vis, acc64=acc64)
         check(lib().mdt_fx_xengine(self._h, _ptr(xin, torch.complex64), _ptr(vis, torch.complex64),
                                    _ptr(acc64, torch.complex128) if acc64 is not None else None))
         return vis
 
     def dump(self, x8, xin, vis, acc64=None):
+        self._check_sizes(x8=x8, xin=xin, vis=vis, acc64=acc64)
         check(lib().mdt_fx_dump(self._h, _ptr(x8, torch.int8), _ptr(xin, torch.complex64),
                                 _ptr(vis, torch.complex64),
                                 _ptr(acc64, torch.complex128) if acc64 is not None else None))

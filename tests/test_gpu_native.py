@@ -33,7 +33,7 @@ def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns):
     print(f"fengine NN={NN} N={N} T={T} Ns={Ns}", flush=True)
     plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_HAMMING)
     x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=N + T, sigma=16.0, tones=((0.1234, 20.0),))
-    xin = plan.fengine(x8, plan.alloc_xinput())
+    xin = plan.xinput_view(plan.fengine(x8, plan.alloc_xinput()))
     W = plan.window()
     F = mdtfx.pps_batch_i8(W, x8, NN, Ns)
     fp = mdtfx.FftPlan(N, Ns * NN)
@@ -61,6 +61,24 @@ def test_native_chain_c1_vs_fp64():
     # FP32 port on the CPU agrees to the same bar
     v32 = orc.fx_chain_f32(W.astype(np.float32), _host(x8), NN, N, T, Ns)
     assert _tri_autonorm_err(v32, v64, NN) < 1e-5
+
+
+@pytest.mark.parametrize("pads", [(0, 0), (8, 64), (2, 2)])
+def test_xinput_padding_layout(pads):
+    """padded X input: same samples at k*chan_pitch + e*row_pitch + s, same vis"""
+    NN, N, T, Ns = 6, 512, 8, 12
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=4, sigma=14.0, rho=0.2)
+    p0 = mdtfx.FxPlan(NN, N, T, Ns)
+    ref = _host(p0.xinput_view(p0.fengine(x8, p0.alloc_xinput())))
+    v0 = _host(p0.xengine(p0.fengine(x8, p0.alloc_xinput()), p0.alloc_vis()))
+    p = mdtfx.FxPlan(NN, N, T, Ns)
+    p.set_xinput_padding(*pads)
+    rp, cp = p.xinput_pitch()
+    assert rp == Ns + (Ns & 1) + pads[0] and cp == NN * rp + pads[1]
+    buf = p.fengine(x8, p.alloc_xinput())
+    assert np.array_equal(_host(p.xinput_view(buf)).view(np.uint32), ref.view(np.uint32))
+    v = _host(p.xengine(buf, p.alloc_vis()))
+    assert np.array_equal(v.view(np.uint32), v0.view(np.uint32))
 
 
 @pytest.mark.parametrize("NN,N,T,Ns", [(3, 32, 4, 2), (65, 256, 4, 70), (130, 128, 8, 40)])
@@ -94,13 +112,13 @@ def test_streaming_history_equals_long_run():
     NN, N, T, Ns, D = 4, 256, 8, 16, 3
     long_x8 = mdtfx.synth_i8(NN, N, D * Ns + T - 1, seed=77, sigma=12.0)
     long_plan = mdtfx.FxPlan(NN, N, T, D * Ns)
-    long_xin = _host(long_plan.fengine(long_x8, long_plan.alloc_xinput()))  # [N][NN][D*Ns]
+    long_xin = _host(long_plan.xinput_view(long_plan.fengine(long_x8, long_plan.alloc_xinput())))
     plan = mdtfx.FxPlan(NN, N, T, Ns)
     buf = torch.empty((Ns + T - 1, NN, N, 2), dtype=torch.int8, device="cuda")
     buf[:T - 1] = long_x8[:T - 1]
     for d in range(D):
         buf[T - 1:] = long_x8[T - 1 + d * Ns: T - 1 + (d + 1) * Ns]
-        xin = _host(plan.fengine(buf, plan.alloc_xinput()))
+        xin = np.ascontiguousarray(_host(plan.xinput_view(plan.fengine(buf, plan.alloc_xinput()))))
         assert np.array_equal(xin.view(np.uint32), np.ascontiguousarray(long_xin[:, :, d * Ns:(d + 1) * Ns]).view(np.uint32))
         mdtfx.history_shift(buf, NN, N, T, Ns)
 
@@ -133,7 +151,8 @@ def test_fp16_split_vs_fp64(NN, N, T, Ns):
     v16 = _host(plan.xengine(xin, torch.full((N, plan.nb), complex(np.nan, np.nan), dtype=torch.complex64, device="cuda")))
     assert np.all(np.isfinite(v16))
     plan.set_precision(mdtfx.XPREC_FP32)
-    v32 = _host(plan.xengine(xin, plan.alloc_vis()))
+    xin32 = plan.fengine(x8, plan.alloc_xinput())
+    v32 = _host(plan.xengine(xin32, plan.alloc_vis()))
     W = _host(plan.window()).astype(np.float64)
     chans = np.unique(np.linspace(0, N - 1, min(N, 24)).astype(int))
     v64 = orc.fx_chain_f64(W, _host(x8), NN, N, T, Ns, chans)
@@ -141,3 +160,43 @@ def test_fp16_split_vs_fp64(NN, N, T, Ns):
     e32 = _tri_autonorm_err(v32[chans], v64, NN)
     print(f"max normalised error vs FP64: fp16-split {e16:.3e}  fp32 {e32:.3e}")
     assert e16 < 1e-5 and e32 < 1e-5
+
+
+def test_fp16_split_epilogue_format():
+    """FP16-split X input: each sample is (hi_re, hi_im, lo_re, lo_im) of x*s,
+    s an exact power of two, hi = fp16(x*s), lo = fp16(x*s - hi)."""
+    NN, N, T, Ns = 5, 1024, 8, 16
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=12, sigma=20.0, tones=((0.2, 60.0),))
+    p32 = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP32)
+    p16 = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
+    x32 = np.ascontiguousarray(_host(p32.xinput_view(p32.fengine(x8, p32.alloc_xinput()))))
+    raw = np.ascontiguousarray(_host(p16.xinput_view(p16.fengine(x8, p16.alloc_xinput()))))
+    h = raw.view(np.float16).reshape(N, NN, Ns, 4).astype(np.float64)
+    hi = h[..., 0] + 1j * h[..., 1]
+    lo = h[..., 2] + 1j * h[..., 3]
+    ref = x32.astype(np.complex128)
+    big = np.abs(ref) > 1e-2 * np.abs(ref).max()
+    s = np.median(np.abs(hi[big]) / np.abs(ref[big]))
+    sc = 2.0 ** np.round(np.log2(s))
+    assert abs(s / sc - 1) < 1e-3
+    # hi is the FP16 rounding of x*s; lo the FP16 rounding of the remainder
+    xs = ref * sc
+    assert np.array_equal(hi.real, xs.real.astype(np.float16).astype(np.float64))
+    assert np.array_equal(hi.imag, xs.imag.astype(np.float16).astype(np.float64))
+    err = np.abs(hi + lo - xs)
+    assert np.max(err[big] / np.abs(xs[big])) < 2.0 ** -20
+    assert np.max(np.abs(xs)) < 32768.0  # headroom below the FP16 maximum
+
+
+def test_front_end_refuses_undersized_buffers():
+    """the kernels take raw pointers; the front end checks extents first"""
+    NN, N, T, Ns = 4, 256, 4, 8
+    plan = mdtfx.FxPlan(NN, N, T, Ns)
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=1)
+    small = torch.empty((N, NN * Ns), dtype=torch.complex64, device="cuda")  # unpadded
+    with pytest.raises(ValueError):
+        plan.fengine(x8, small)
+    with pytest.raises(ValueError):
+        plan.fengine(x8[:-1], plan.alloc_xinput())
+    with pytest.raises(ValueError):
+        plan.xengine(plan.alloc_xinput(), torch.empty((N, plan.nb - 1), dtype=torch.complex64, device="cuda"))
