@@ -116,9 +116,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-spectra", type=int, default=256)
+    ap.add_argument("--cpu-spectra", type=int, default=2048)
     ap.add_argument("--err-channels", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 error legs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures a single MI355X; --gpus {args.gpus} is not supported "
@@ -131,6 +133,15 @@ def main():
     import mdtfx
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
+        return
     stream = torch.cuda.current_stream().cuda_stream
     log(f"bench: C3 NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
     plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)

@@ -22,7 +22,9 @@ namespace mdt {
 // corner-turn method)
 constexpr int FE_SB = 4;
 constexpr int FE_NX = 2;
-constexpr bool FE_DPPW = false;
+constexpr int FE_WMODE = 0;
+constexpr bool FE_WPERSIST = true;
+constexpr int FE_MR = 1;
 
 template <int N>
 struct FCfg {
@@ -185,10 +187,14 @@ __device__ __forceinline__ float swap_pair(float v) {  // lane l <-> l ^ 1 (DPP 
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 
-// NX: transforms per LDS exchange round.  DPPW: corner turn through lane-pair
-// DPP swaps (SB = 4) instead of LDS staging.
-template <int N, int T, int SB, int NX, bool DPPW>
+// NX: transforms per LDS exchange round.  WMODE: corner turn through LDS
+// staging (0), lane-pair DPP swaps with SB = 4 (1), or each lane storing its
+// own SB-spectrum segment (2).  WPERSIST: taps stay in registers for the
+// whole kernel (true) or are reloaded (L2-resident) for each item and are dead
+// outside the FIR, which frees registers for larger SB.
+template <int N, int T, int SB, int NX, int WMODE, bool WPERSIST, int MR>
 __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, long nitems) {
+  constexpr bool DPPW = (WMODE == 1);
   static_assert(!DPPW || SB == 4, "the lane-pair corner turn writes 4 spectra per segment");
   static_assert(SB % NX == 0, "spectra per block must be a multiple of NX");
   constexpr int P = 8, TPB = N / P;
@@ -198,7 +204,8 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
   constexpr int TWN = pass_tw_entries<N, P>();
   constexpr int ROW = SB + 1;
   constexpr int FBUF = NX * lds_entries<N>();
-  constexpr int BUF = (DPPW || FBUF >= TPB * ROW) ? FBUF : TPB * ROW;
+  constexpr int SBUF = MR * TPB * ROW;  // staging for MR channel rounds
+  constexpr int BUF = (WMODE != 0 || FBUF >= SBUF) ? FBUF : SBUF;
   __shared__ float2 twl[TWN > 0 ? TWN : 1];
   __shared__ float2 buf[BUF];
   const int t = threadIdx.x;
@@ -216,10 +223,12 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
     twl[idx] = (r < R) ? a.tw[jm * r * (N / (ns * R))] : make_float2(0.f, 0.f);
   }
   float w[T][P];
+  if constexpr (WPERSIST) {
 #pragma unroll
-  for (int i = 0; i < T; i++)
+    for (int i = 0; i < T; i++)
 #pragma unroll
-    for (int m = 0; m < P; m++) w[i][m] = a.W[(long)i * N + t + m * TPB];
+      for (int m = 0; m < P; m++) w[i][m] = a.W[(long)i * N + t + m * TPB];
+  }
   __syncthreads();
   const PassTw<P> tws{twl};
 
@@ -238,6 +247,12 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
     const int scount = min(SB, a.Ns - s0);
     const char2 *src = a.in + (long)s0 * spec + (long)e * N + t;
 
+    if constexpr (!WPERSIST) {
+#pragma unroll
+      for (int i = 0; i < T; i++)
+#pragma unroll
+        for (int m = 0; m < P; m++) w[i][m] = a.W[(long)i * N + t + m * TPB];
+    }
     float2 acc[SB][P];
 #pragma unroll
     for (int sl = 0; sl < SB; sl++)
@@ -289,7 +304,28 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
       }
     }
 
-    if constexpr (DPPW) {
+    if constexpr (WMODE == 2) {
+      // each lane stores its own channels' SB-spectrum segments (16-byte
+      // stores), no LDS staging and no barriers
+      const bool full = (scount == SB) && ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        const int k = t + m * TPB;
+        float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0;
+        float2 o[SB];
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) o[sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
+        if (full) {
+#pragma unroll
+          for (int sl = 0; sl < SB; sl += 2)
+            *(float4 *)(dst + sl) = make_float4(o[sl].x, o[sl].y, o[sl + 1].x, o[sl + 1].y);
+        } else {
+#pragma unroll
+          for (int sl = 0; sl < SB; sl++)
+            if (sl < scount) dst[sl] = o[sl];
+        }
+      }
+    } else if constexpr (DPPW) {
       // corner turn without LDS: lanes 2g and 2g+1 hold channels k and k+1 for
       // 4 spectra (two 16-byte chunks each); one DPP swap gives each lane pair
       // the two halves of one channel's 32-byte segment for each of 2 stores.
@@ -319,50 +355,57 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
         }
       }
     } else {
-      // corner turn staged per m through LDS: each store instruction writes
-      // SB consecutive spectra of a (channel, input) row per SB/2 lanes
+      // corner turn staged through LDS, MR channel rounds per barrier pair:
+      // each store instruction writes SB consecutive spectra of a (channel,
+      // input) row per SB/2 lanes
       constexpr int CH = SB / 2;
       const bool vec = ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
 #pragma unroll
-      for (int m = 0; m < P; m++) {
+      for (int m0 = 0; m0 < P; m0 += MR) {
         __syncthreads();
 #pragma unroll
-        for (int sl = 0; sl < SB; sl++)
-          buf[t * ROW + sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
+        for (int mr = 0; mr < MR; mr++)
+#pragma unroll
+          for (int sl = 0; sl < SB; sl++)
+            buf[mr * TPB * ROW + t * ROW + sl] =
+                a.split ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
         __syncthreads();
 #pragma unroll
-        for (int v = 0; v < CH; v++) {
-          const int q = t + v * TPB;
-          const int tt = q / CH, pr = q % CH;
-          const int k = tt + m * TPB;
-          const int sl = 2 * pr;
-          float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0 + sl;
-          const float2 v0 = buf[tt * ROW + sl], v1 = buf[tt * ROW + sl + 1];
-          if (vec && sl + 1 < scount) {
-            *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
-          } else {
-            if (sl < scount) dst[0] = v0;
-            if (sl + 1 < scount) dst[1] = v1;
+        for (int mr = 0; mr < MR; mr++)
+#pragma unroll
+          for (int v = 0; v < CH; v++) {
+            const int q = t + v * TPB;
+            const int tt = q / CH, pr = q % CH;
+            const int k = tt + (m0 + mr) * TPB;
+            const int sl = 2 * pr;
+            float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0 + sl;
+            const float2 *src_ = buf + mr * TPB * ROW + tt * ROW;
+            const float2 v0 = src_[sl], v1 = src_[sl + 1];
+            if (vec && sl + 1 < scount) {
+              *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+            } else {
+              if (sl < scount) dst[0] = v0;
+              if (sl + 1 < scount) dst[1] = v1;
+            }
           }
-        }
       }
       __syncthreads();  // the staging buffer is the next item's FFT buffer
     }
   }
 }
 
-template <int N, int T, int SB, int NX, bool DPPW>
+template <int N, int T, int SB, int NX, int WMODE, bool WPERSIST, int MR>
 static int launch_persistent(const FArgs &a, hipStream_t st) {
   const long nitems = (long)a.NN * ((a.Ns + SB - 1) / SB);
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SB, NX, DPPW>, N / 8,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SB, NX, WMODE, WPERSIST, MR>, N / 8,
                                                    0) != hipSuccess)
     return fail(MDT_ERR_DEVICE, "device attributes for the F-engine grid");
   long grid = (long)cus * (per_cu > 0 ? per_cu : 1);
   if (grid > nitems) grid = nitems;
-  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SB, NX, DPPW>), dim3((unsigned)grid), dim3(N / 8), 0, st, a,
+  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SB, NX, WMODE, WPERSIST, MR>), dim3((unsigned)grid), dim3(N / 8), 0, st, a,
                      nitems);
   return check_launch("fengine_persistent_kernel");
 }
@@ -371,8 +414,8 @@ template <int N>
 static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   using C = FCfg<N>;
   if constexpr (N >= 512 && N <= 4096) {
-    if (a.T == 4) return launch_persistent<N, 4, FE_SB, FE_NX, FE_DPPW>(a, st);
-    if (a.T == 8) return launch_persistent<N, 8, FE_SB, FE_NX, FE_DPPW>(a, st);
+    if (a.T == 4) return launch_persistent<N, 4, FE_SB, FE_NX, FE_WMODE, FE_WPERSIST, FE_MR>(a, st);
+    if (a.T == 8) return launch_persistent<N, 8, FE_SB, FE_NX, FE_WMODE, FE_WPERSIST, FE_MR>(a, st);
   }
   const long nsb = (a.Ns + C::SB - 1) / C::SB;
   const long neg = (a.NN + C::EB - 1) / C::EB;
