@@ -185,11 +185,35 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
  * hi = fp16(x*s), lo = fp16(x*s - hi).  Run the F-engine at the precision the
  * X-engine will use. */
 int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
+/* Per-input complex gain g_e and delay tau_e (samples) applied by the F-engine
+ * to each channel k: X'_e[k] = g_e exp(-2 pi i k_s tau_e / N) X_e[k], k_s the
+ * signed bin.  Device arrays of NN entries, read at every F-engine launch;
+ * gains = NULL switches the stage off.  No counterpart in the reference
+ * (calibration extension). */
+int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *delays);
 int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_complex *Vis,
                    double *Acc64);
 int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative,
                 mdt_complex *Vis, double *Acc64);
 int mdt_fx_plan_destroy(mdt_fx_plan plan);
+
+/* ---------------------------------------------------------------- calibration (a9)
+ * No counterpart in the reference.  For raw triangle visibilities V (as from
+ * mdt_fx_xengine) and per-input gains g, delays tau: G_a(c) = g_a
+ * exp(-2 pi i k_c tau_a / N), V'_ab = G_a conj(G_b) V_ab (what the F-engine
+ * gain stage produces), L = sum_{c, a>=b} |V' - M|^2.  mdt_calib_loss_grad
+ * writes L and the adjoint grad_g = dL/dRe g + i dL/dIm g, grad_tau = dL/dtau
+ * (each output nullable, grad_tau needs grad_g).  FP32 with compensated,
+ * fixed-order (deterministic) sums; the plan owns its workspace. */
+typedef struct mdt_calib_plan_s *mdt_calib_plan;
+int mdt_calib_plan_create(mdt_calib_plan *plan, int NpolsxNelements, int Nchannels);
+int mdt_calib_set_stream(mdt_calib_plan plan, hipStream_t stream);
+int mdt_calib_apply(mdt_calib_plan plan, const mdt_complex *V, const mdt_complex *g, const float *tau,
+                    mdt_complex *Vp);
+int mdt_calib_loss_grad(mdt_calib_plan plan, const mdt_complex *V, const mdt_complex *M,
+                        const mdt_complex *g, const float *tau, float *loss, mdt_complex *grad_g,
+                        float *grad_tau);
+int mdt_calib_plan_destroy(mdt_calib_plan plan);
 
 /* ---------------------------------------------------------------- synthetic input
  * Deterministic counter-based generator of int8 (I, Q) samples for the

@@ -13,6 +13,7 @@
 // mdt_fft_exec_c2c), and the SB x N results are corner-turned through LDS so
 // that each store instruction writes 64-byte runs of SB consecutive spectra of
 // one (channel, input) row of the channel-major output [N][NN][Ns].
+#include "calib.hpp"
 #include "fengine.hpp"
 #include "fft_core.hpp"
 
@@ -128,10 +129,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
 #pragma unroll
     for (int m = 0; m < P; m++) {
       const int k = t + m * TPB;
-      const int kk = (k < N / 2) ? k : k - N;
-      float sn, cs;
-      sincospif(-2.0f * (float)kk * tau / (float)N, &sn, &cs);
-      const float2 ph = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
+      const float2 ph = cmul(g, delay_phasor(signed_bin(k, N), tau, N));
 #pragma unroll
       for (int sl = 0; sl < SB; sl++) acc[sl][m] = cmul(acc[sl][m], ph);
     }
@@ -295,10 +293,7 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
 #pragma unroll
       for (int m = 0; m < P; m++) {
         const int k = t + m * TPB;
-        const int kk = (k < N / 2) ? k : k - N;
-        float sn, cs;
-        sincospif(-2.0f * (float)kk * tau / (float)N, &sn, &cs);
-        const float2 ph = make_float2(g.x * cs - g.y * sn, g.x * sn + g.y * cs);
+        const float2 ph = cmul(g, delay_phasor(signed_bin(k, N), tau, N));
 #pragma unroll
         for (int sl = 0; sl < SB; sl++) acc[sl][m] = cmul(acc[sl][m], ph);
       }

@@ -105,6 +105,23 @@ int mdt_fx_set_precision(mdt_fx_plan plan, int xprec) {
   return MDT_SUCCESS;
 }
 
+int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *delays) {
+  MDT_REQUIRE_PTR(plan);
+  if (!gains && delays) return fail(MDT_ERR_NULL_POINTER, "delays need gains");
+  float gmax = 1.0f;
+  if (gains) {
+    std::vector<float2> h(plan->NN);
+    if (hipMemcpy(h.data(), gains, sizeof(float2) * plan->NN, hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(MDT_ERR_DEVICE, "gain read-back");
+    gmax = 0.0f;
+    for (const float2 &v : h) gmax = std::fmax(gmax, std::sqrt(v.x * v.x + v.y * v.y));
+    gmax = std::fmax(gmax, 1.0f);
+  }
+  plan->gain = (const float2 *)gains;
+  plan->delay = delays;
+  return choose_f16_scale(plan, gmax);  // the split scale must cover |g| x bound
+}
+
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(bytes);

@@ -95,6 +95,12 @@ _SIGS = {
     "mdt_fx_dump": [_VP, _VP, _VP, _VP, _VP],
     "mdt_fx_plan_destroy": [_VP],
     "mdt_synth_i8": [ctypes.POINTER(SynthParams), _I, _I, _I, _VP, _VP],
+    "mdt_fx_set_gains": [_VP, _VP, _VP],
+    "mdt_calib_plan_create": [ctypes.POINTER(_VP), _I, _I],
+    "mdt_calib_set_stream": [_VP, _VP],
+    "mdt_calib_apply": [_VP, _VP, _VP, _VP, _VP],
+    "mdt_calib_loss_grad": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
+    "mdt_calib_plan_destroy": [_VP],
 }
 
 
@@ -273,6 +279,18 @@ class FxPlan:
 
     def set_precision(self, xprec):
         check(lib().mdt_fx_set_precision(self._h, xprec))
+
+    def set_gains(self, gains=None, delays=None):
+        """Per-input complex gains (complex64 [NN]) and delays (float32 [NN],
+        samples) applied by the F-engine; None switches the stage off.  The
+        tensors must stay alive while the plan uses them."""
+        if gains is not None and gains.numel() < self.NN:
+            raise ValueError("gains need NN entries")
+        if delays is not None and delays.numel() < self.NN:
+            raise ValueError("delays need NN entries")
+        self._gains = (gains, delays)
+        check(lib().mdt_fx_set_gains(self._h, _ptr(gains, torch.complex64) if gains is not None else None,
+                                     _ptr(delays, torch.float32) if delays is not None else None))
 
     def set_stream(self, stream_handle):
         check(lib().mdt_fx_set_stream(self._h, ctypes.c_void_p(stream_handle)))

@@ -1,0 +1,129 @@
+"""Gain/delay calibration (operating point C5: 64 inputs, 4-tap PFB, 1024
+channels, 512 spectra): the F-engine gain stage, the GPU loss and the
+hand-written adjoint, against the FP64 oracle (analytic gradient and central
+finite differences)."""
+import numpy as np
+import pytest
+import torch
+
+import mdtfx
+import oracle as orc
+from mdtfx.calib import CalibPlan, calib_loss
+
+pytestmark = pytest.mark.gpu
+
+NN, N, T, NS = 64, 1024, 4, 512
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def c5():
+    rng = np.random.default_rng(55)
+    plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP32)
+    x8 = mdtfx.synth_i8(NN, N, NS + T - 1, seed=55, sigma=16.0, rho=0.5, max_delay=4)
+    V = plan.dump(x8, plan.alloc_xinput(), plan.alloc_vis())
+    g_true = (rng.uniform(0.8, 1.2, NN) * np.exp(1j * rng.uniform(-np.pi, np.pi, NN))).astype(np.complex64)
+    tau_true = rng.uniform(-4, 4, NN).astype(np.float32)
+    g = (g_true * (1 + 0.05 * (rng.standard_normal(NN) + 1j * rng.standard_normal(NN)))).astype(np.complex64)
+    tau = (tau_true + 0.1 * rng.standard_normal(NN)).astype(np.float32)
+    cal = CalibPlan(NN, N)
+    M = cal.apply(V, torch.from_numpy(g_true).cuda(), torch.from_numpy(tau_true).cuda())
+    return dict(plan=plan, x8=x8, V=V, M=M, g=g, tau=tau, g_true=g_true, tau_true=tau_true, cal=cal)
+
+
+def _autonorm(v, v64):
+    a, b = np.tril_indices(NN)
+    diag = np.array([orc.tri_index(i, i) for i in range(NN)])
+    auto = np.abs(np.real(v64[:, diag]))
+    return float(np.max(np.abs(v - v64) / np.sqrt(auto[:, a] * auto[:, b])))
+
+
+def test_fengine_gain_stage_matches_apply(c5):
+    """V' from the gained F-engine == G_a conj(G_b) V (FP32 on both sides)"""
+    plan = c5["plan"]
+    g = torch.from_numpy(c5["g"]).cuda()
+    tau = torch.from_numpy(c5["tau"]).cuda()
+    plan.set_gains(g, tau)
+    Vfe = _host(plan.dump(c5["x8"], plan.alloc_xinput(), plan.alloc_vis()))
+    plan.set_gains(None, None)
+    Vap = _host(c5["cal"].apply(c5["V"], g, tau))
+    V64 = orc.calib_apply(_host(c5["V"]).astype(np.complex128), c5["g"].astype(np.complex128),
+                          c5["tau"].astype(np.float64), NN, N)
+    assert _autonorm(Vap, V64) < 1e-6
+    assert _autonorm(Vfe, V64) < 1e-5
+
+
+def test_loss_and_adjoint_vs_fp64(c5):
+    V = c5["V"]
+    V64 = _host(V).astype(np.complex128)
+    M64 = _host(c5["M"]).astype(np.complex128)
+    g64 = c5["g"].astype(np.complex128)
+    t64 = c5["tau"].astype(np.float64)
+    L64, gg64, gt64 = orc.calib_loss_grad(V64, M64, g64, t64, NN, N)
+    loss, gg, gt = c5["cal"].loss_grad(V, c5["M"], torch.from_numpy(c5["g"]).cuda(),
+                                       torch.from_numpy(c5["tau"]).cuda())
+    L, gg, gt = float(_host(loss)), _host(gg), _host(gt)
+    assert abs(L - L64) / L64 < 1e-6, (L, L64)
+    eg = np.max(np.abs(gg - gg64)) / np.max(np.abs(gg64))
+    et = np.max(np.abs(gt - gt64)) / np.max(np.abs(gt64))
+    print(f"adjoint vs FP64 analytic: gain {eg:.2e}, delay {et:.2e}")
+    assert eg < 1e-6
+    assert et < 1e-5
+
+
+def test_adjoint_vs_finite_differences(c5):
+    """GPU adjoint vs FP64 central finite differences of the FP64 loss"""
+    V64 = _host(c5["V"]).astype(np.complex128)
+    M64 = _host(c5["M"]).astype(np.complex128)
+    g64 = c5["g"].astype(np.complex128)
+    t64 = c5["tau"].astype(np.float64)
+    _, gg, gt = c5["cal"].loss_grad(c5["V"], c5["M"], torch.from_numpy(c5["g"]).cuda(),
+                                    torch.from_numpy(c5["tau"]).cuda())
+    gg, gt = _host(gg), _host(gt)
+    h = 1e-5
+    sel = np.arange(0, NN, 5)  # 13 of the 64 inputs: 78 FP64 loss evaluations
+    fd_g = np.zeros(NN, np.complex128)
+    fd_t = np.zeros(NN)
+    L = lambda g, t: orc.calib_loss_grad(V64, M64, g, t, NN, N)[0]
+    for a in sel:
+        for unit in (1.0, 1j):
+            gp, gm = g64.copy(), g64.copy()
+            gp[a] += h * unit
+            gm[a] -= h * unit
+            d = (L(gp, t64) - L(gm, t64)) / (2 * h)
+            fd_g[a] += d * (1.0 if unit == 1.0 else 1j)
+        tp, tm = t64.copy(), t64.copy()
+        tp[a] += h
+        tm[a] -= h
+        fd_t[a] = (L(g64, tp) - L(g64, tm)) / (2 * h)
+    eg = np.max(np.abs(gg[sel] - fd_g[sel])) / np.max(np.abs(fd_g[sel]))
+    et = np.max(np.abs(gt[sel] - fd_t[sel])) / np.max(np.abs(fd_t[sel]))
+    print(f"adjoint vs FP64 finite differences: gain {eg:.2e}, delay {et:.2e}")
+    assert eg < 1e-6
+    assert et < 1e-5
+
+
+def test_autograd_function(c5):
+    g = torch.from_numpy(c5["g"]).cuda().requires_grad_(True)
+    tau = torch.from_numpy(c5["tau"]).cuda().requires_grad_(True)
+    loss = calib_loss(g, tau, c5["V"], c5["M"], c5["cal"])
+    (3.0 * loss).backward()
+    _, gg, gt = c5["cal"].loss_grad(c5["V"], c5["M"], g.detach(), tau.detach())
+    assert torch.allclose(g.grad, 3.0 * gg) and torch.allclose(tau.grad, 3.0 * gt)
+    # a few gradient-descent steps lower the loss
+    with torch.no_grad():
+        l0 = float(loss)
+        g2 = g - 1e-3 * g.grad / g.grad.abs().max()
+        t2 = tau - 1e-3 * tau.grad / tau.grad.abs().max()
+        l1 = float(calib_loss(g2, t2, c5["V"], c5["M"], c5["cal"]))
+    assert l1 < l0
+
+
+def test_loss_zero_at_truth(c5):
+    loss, _, _ = c5["cal"].loss_grad(c5["V"], c5["M"], torch.from_numpy(c5["g_true"]).cuda(),
+                                     torch.from_numpy(c5["tau_true"]).cuda(), want_grad=False)
+    assert float(_host(loss)) == 0.0
