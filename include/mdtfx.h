@@ -73,6 +73,7 @@ enum mdt_status {
 /* X-engine precisions of the native path */
 #define MDT_XPREC_FP32 0       /* v_mfma_f32_32x32x2f32 */
 #define MDT_XPREC_FP16_SPLIT 1 /* hi+lo FP16, three v_mfma_f32_32x32x16_f16 products */
+#define MDT_XPREC_FP8 2        /* e4m3 with per-channel power-of-two RMS scaling, v_mfma_scale 32x32x64 */
 
 const char *mdt_last_error_string(void);
 int mdt_version(int *version);
@@ -182,8 +183,10 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
 /* The X input's sample format follows the plan's precision: FP32 (re, im) for
  * MDT_XPREC_FP32; for MDT_XPREC_FP16_SPLIT each 8-byte sample holds FP16
  * (hi_re, hi_im, lo_re, lo_im) of x*s, s the plan's power-of-two scale,
- * hi = fp16(x*s), lo = fp16(x*s - hi).  Run the F-engine at the precision the
- * X-engine will use. */
+ * hi = fp16(x*s), lo = fp16(x*s - hi); for MDT_XPREC_FP8 each sample is 4
+ * bytes, FP16 (re, im) of x*s, and the F-engine also sums each channel's power
+ * (FP8 needs Nchannels in [512, 4096] and Ntaps 4 or 8).  Run the F-engine at
+ * the precision the X-engine will use. */
 int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
 /* Per-input complex gain g_e and delay tau_e (samples) applied by the F-engine
  * to each channel k: X'_e[k] = g_e exp(-2 pi i k_s tau_e / N) X_e[k], k_s the
@@ -195,6 +198,11 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
                    double *Acc64);
 int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative,
                 mdt_complex *Vis, double *Acc64);
+/* FP8 plans: the per-channel exponents e_c the X-engine applies (device int
+ * array of Nchannels): samples are quantised as e4m3(x*s*2^e_c), RMS ~ 32. */
+int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents);
+/* the plan's power-of-two scale s of the FP16 sample formats */
+int mdt_fx_get_split_scale(mdt_fx_plan plan, float *scale);
 int mdt_fx_plan_destroy(mdt_fx_plan plan);
 
 /* ---------------------------------------------------------------- calibration (a9)

@@ -234,6 +234,9 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
   const long G = gridDim.x;
   const long b = blockIdx.x;
   const long spec = (long)a.NN * N;
+  float pw[P];  // split 2: this thread's channels' power over all its items
+#pragma unroll
+  for (int m = 0; m < P; m++) pw[m] = 0.f;
   for (long base = 0; base < nitems; base += G) {
     // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
     // spectrum blocks of one input) go to the same XCD in each round.
@@ -297,6 +300,17 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
 #pragma unroll
         for (int sl = 0; sl < SB; sl++) acc[sl][m] = cmul(acc[sl][m], ph);
       }
+    }
+
+    if (a.split == 2) {
+#pragma unroll
+      for (int m = 0; m < P; m++)
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++)
+          if (sl < scount) {
+            const float xr = acc[sl][m].x * a.split_scale, xi = acc[sl][m].y * a.split_scale;
+            pw[m] = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, pw[m]));
+          }
     }
 
     if constexpr (WMODE == 2) {
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
             buf[mr * TPB * ROW + t * ROW + sl] =
-                a.split ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
+                (a.split == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
         __syncthreads();
 #pragma unroll
         for (int mr = 0; mr < MR; mr++)
@@ -373,10 +387,19 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
             const int tt = q / CH, pr = q % CH;
             const int k = tt + (m0 + mr) * TPB;
             const int sl = 2 * pr;
-            float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0 + sl;
+            const long o = (long)k * a.cp + (long)e * a.rp + s0 + sl;
+            float2 *dst = a.out + o;
             const float2 *src_ = buf + mr * TPB * ROW + tt * ROW;
             const float2 v0 = src_[sl], v1 = src_[sl + 1];
-            if (vec && sl + 1 < scount) {
+            if (a.split == 2) {  // 4-byte FP16 samples
+              unsigned *d4 = (unsigned *)a.out + o;
+              if (vec && sl + 1 < scount) {
+                *(uint2 *)d4 = make_uint2(half_sample(v0, a.split_scale), half_sample(v1, a.split_scale));
+              } else {
+                if (sl < scount) d4[0] = half_sample(v0, a.split_scale);
+                if (sl + 1 < scount) d4[1] = half_sample(v1, a.split_scale);
+              }
+            } else if (vec && sl + 1 < scount) {
               *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
             } else {
               if (sl < scount) dst[0] = v0;
@@ -386,6 +409,10 @@ __global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, l
       }
       __syncthreads();  // the staging buffer is the next item's FFT buffer
     }
+  }
+  if (a.split == 2) {
+#pragma unroll
+    for (int m = 0; m < P; m++) atomicAdd(&a.power[t + m * TPB], pw[m]);
   }
 }
 
@@ -409,9 +436,12 @@ template <int N>
 static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   using C = FCfg<N>;
   if constexpr (N >= 512 && N <= 4096) {
+    if (a.split == 2 && FE_WMODE != 0) return fail(MDT_ERR_UNSUPPORTED, "FP8 needs the LDS corner turn");
     if (a.T == 4) return launch_persistent<N, 4, FE_SB, FE_NX, FE_WMODE, FE_WPERSIST, FE_MR>(a, st);
     if (a.T == 8) return launch_persistent<N, 8, FE_SB, FE_NX, FE_WMODE, FE_WPERSIST, FE_MR>(a, st);
   }
+  if (a.split == 2)
+    return fail(MDT_ERR_UNSUPPORTED, "FP8 X-engine input needs Nchannels in [512, 4096] and Ntaps 4 or 8");
   const long nsb = (a.Ns + C::SB - 1) / C::SB;
   const long neg = (a.NN + C::EB - 1) / C::EB;
   const long total = nsb * neg;

@@ -14,9 +14,18 @@ struct FArgs {
   long rp, cp;          // row pitch (>= Ns) and channel pitch (>= NN*rp), in samples
   const float2 *gain;   // optional [NN] complex gains
   const float *delay;   // optional [NN] delays in samples
-  int split;            // 0: FP32 (re, im); 1: FP16 (hi_re, hi_im, lo_re, lo_im) of x*split_scale
+  int split;            // 0: FP32 (re, im); 1: FP16 (hi_re, hi_im, lo_re, lo_im) of x*split_scale;
+                        // 2: FP16 (re, im) of x*split_scale in 4-byte elements + channel power
   float split_scale;    // exact power of two
+  float *power;         // split 2: per-channel sum of |x*split_scale|^2 (zeroed by the caller)
 };
+
+// FP16 (re, im) of a sample scaled by an exact power of two, as 4 bytes
+__device__ __forceinline__ unsigned half_sample(float2 v, float sc) {
+  const _Float16 r = (_Float16)(v.x * sc), i = (_Float16)(v.y * sc);
+  return (unsigned)__builtin_bit_cast(unsigned short, r) |
+         ((unsigned)__builtin_bit_cast(unsigned short, i) << 16);
+}
 
 // FP16 hi/lo split of one complex sample scaled by an exact power of two:
 // hi = fp16(x*sc), lo = fp16(x*sc - hi); 8 bytes (hi_re, hi_im, lo_re, lo_im).

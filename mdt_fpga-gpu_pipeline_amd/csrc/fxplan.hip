@@ -21,9 +21,18 @@ struct mdt_fx_plan_s {
   const float2 *gain = nullptr;
   const float *delay = nullptr;
   float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15
+  float *power = nullptr;  // FP8 path: per-channel power from the F-engine
   int row_pad = 8, chan_pad = 64;  // X input padding (samples) after each row / channel
-  long rp() const { return (long)Ns + (Ns & 1) + row_pad; }  // even: 16-byte rows
-  long cp() const { return (long)NN * rp() + chan_pad; }
+  // pitches in samples; 16-byte aligned rows: even for 8-byte samples, a
+  // multiple of 4 for the 4-byte FP8-path samples
+  long rp() const {
+    const long r = (long)Ns + (Ns & 1) + row_pad;
+    return xprec == MDT_XPREC_FP8 ? (r + 3) & ~3L : r;
+  }
+  long cp() const {
+    const long c = (long)NN * rp() + chan_pad;
+    return xprec == MDT_XPREC_FP8 ? (c + 3) & ~3L : c;
+  }
 };
 
 using namespace mdt;
@@ -44,9 +53,19 @@ static int choose_f16_scale(mdt_fx_plan p, float gmax = 1.0f) {
   return MDT_SUCCESS;
 }
 
-static int check_xprec(int xprec) {
-  if (xprec != MDT_XPREC_FP32 && xprec != MDT_XPREC_FP16_SPLIT)
+__global__ void zero_kernel(float *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+static size_t elem_bytes(const mdt_fx_plan_s *p) { return p->xprec == MDT_XPREC_FP8 ? 4 : 8; }
+
+static int check_xprec(int xprec, int N, int T) {
+  if (xprec != MDT_XPREC_FP32 && xprec != MDT_XPREC_FP16_SPLIT && xprec != MDT_XPREC_FP8)
     return fail(MDT_ERR_INVALID_VALUE, "unknown X-engine precision %d", xprec);
+  // the FP8 sample format and channel powers come from the persistent F-engine
+  if (xprec == MDT_XPREC_FP8 && (N < 512 || N > 4096 || (T != 4 && T != 8)))
+    return fail(MDT_ERR_UNSUPPORTED, "FP8 path needs Nchannels in [512, 4096] and Ntaps 4 or 8");
   return MDT_SUCCESS;
 }
 
@@ -60,11 +79,11 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
   MDT_TRY(check_channels(Nchannels));
   MDT_TRY(check_taps(Ntaps));
   MDT_TRY(check_spectra(Nspectra));
-  MDT_TRY(check_xprec(xprec));
   if (!is_pow2(Nchannels))
     return fail(MDT_ERR_UNSUPPORTED, "Nchannels=%d: the FFT needs a power of two", Nchannels);
   if (window_kind != MDT_WINDOW_BH4 && window_kind != MDT_WINDOW_HAMMING)
     return fail(MDT_ERR_INVALID_VALUE, "unknown window kind %d", window_kind);
+  MDT_TRY(check_xprec(xprec, Nchannels, Ntaps));
   mdt_fx_plan p = new (std::nothrow) mdt_fx_plan_s;
   if (!p) return fail(MDT_ERR_ALLOC, "plan");
   p->NN = NpolsxNelements;
@@ -77,13 +96,23 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
     delete p;
     return fail(MDT_ERR_ALLOC, "window table");
   }
+  if (hipMalloc(&p->power, sizeof(float) * (size_t)Nchannels) != hipSuccess) {
+    (void)hipFree(p->W);
+    delete p;
+    return fail(MDT_ERR_ALLOC, "channel power");
+  }
   int st = launch_window(Nchannels, Ntaps, p->W, window_kind, 0);
+  if (st == MDT_SUCCESS) {
+    hipLaunchKernelGGL(zero_kernel, dim3((Nchannels + 255) / 256), dim3(256), 0, 0, p->power, Nchannels);
+    st = check_launch("zero_kernel");
+  }
   if (st == MDT_SUCCESS) st = make_twiddles(Nchannels, &p->tw);
   if (st == MDT_SUCCESS && hipStreamSynchronize(0) != hipSuccess)
     st = fail(MDT_ERR_DEVICE, "plan table build");
   if (st == MDT_SUCCESS) st = choose_f16_scale(p);
   if (st != MDT_SUCCESS) {
     (void)hipFree(p->W);
+    (void)hipFree(p->power);
     if (p->tw) (void)hipFree(p->tw);
     delete p;
     return st;
@@ -100,7 +129,7 @@ int mdt_fx_set_stream(mdt_fx_plan plan, hipStream_t stream) {
 
 int mdt_fx_set_precision(mdt_fx_plan plan, int xprec) {
   MDT_REQUIRE_PTR(plan);
-  MDT_TRY(check_xprec(xprec));
+  MDT_TRY(check_xprec(xprec, plan->N, plan->T));
   plan->xprec = xprec;
   return MDT_SUCCESS;
 }
@@ -125,7 +154,7 @@ int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *de
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(bytes);
-  *bytes = sizeof(float2) * (size_t)plan->N * (size_t)plan->cp();
+  *bytes = elem_bytes(plan) * (size_t)plan->N * (size_t)plan->cp();
   return MDT_SUCCESS;
 }
 
@@ -174,8 +203,14 @@ int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInput
   a.cp = plan->cp();
   a.gain = plan->gain;
   a.delay = plan->delay;
-  a.split = (plan->xprec == MDT_XPREC_FP16_SPLIT) ? 1 : 0;
+  a.split = (plan->xprec == MDT_XPREC_FP16_SPLIT) ? 1 : (plan->xprec == MDT_XPREC_FP8 ? 2 : 0);
   a.split_scale = plan->f16_scale;
+  a.power = plan->power;
+  if (a.split == 2) {  // graph-safe zeroing (a kernel, not a memset node)
+    hipLaunchKernelGGL(zero_kernel, dim3((plan->N + 255) / 256), dim3(256), 0, plan->stream, plan->power,
+                       plan->N);
+    MDT_TRY(check_launch("zero_kernel"));
+  }
   return launch_fengine(a, plan->N, plan->stream);
 }
 
@@ -201,6 +236,7 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
   g.prec = plan->xprec;
   g.in_scale = plan->f16_scale;
   if (plan->xprec == MDT_XPREC_FP16_SPLIT) return xeng_launch_f16split(g, plan->stream);
+  if (plan->xprec == MDT_XPREC_FP8) return xeng_launch_f8(g, plan->power, plan->stream);
   return xeng_launch(g, XL_MK, XO_TRI, plan->stream);
 }
 
@@ -210,9 +246,25 @@ int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNat
   return mdt_fx_xengine(plan, XInputNative, Vis, Acc64);
 }
 
+int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(exponents);
+  if (plan->xprec != MDT_XPREC_FP8) return fail(MDT_ERR_INVALID_VALUE, "plan precision is not FP8");
+  return launch_chan_exponents(plan->N, plan->power, (float)plan->NN * (float)plan->Ns, exponents,
+                               plan->stream);
+}
+
+int mdt_fx_get_split_scale(mdt_fx_plan plan, float *scale) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(scale);
+  *scale = plan->f16_scale;
+  return MDT_SUCCESS;
+}
+
 int mdt_fx_plan_destroy(mdt_fx_plan plan) {
   if (!plan) return MDT_SUCCESS;
   if (plan->W) (void)hipFree(plan->W);
+  if (plan->power) (void)hipFree(plan->power);
   if (plan->tw) (void)hipFree(plan->tw);
   delete plan;
   return MDT_SUCCESS;

@@ -35,5 +35,7 @@ struct XArgs {
 
 int xeng_launch(const XArgs &g, int layout, int out, hipStream_t st);
 int xeng_launch_f16split(const XArgs &g, hipStream_t st);
+int xeng_launch_f8(const XArgs &g, const float *power, hipStream_t st);
+int launch_chan_exponents(int N, const float *power, float count, int *e, hipStream_t st);
 
 }  // namespace mdt

@@ -42,6 +42,7 @@ FFT_C2C = 0x29
 OP_N, OP_T, OP_C = 0, 1, 2
 XPREC_FP32 = 0
 XPREC_FP16_SPLIT = 1
+XPREC_FP8 = 2
 
 _lib = None
 
@@ -94,6 +95,8 @@ _SIGS = {
     "mdt_fx_xengine": [_VP, _VP, _VP, _VP],
     "mdt_fx_dump": [_VP, _VP, _VP, _VP, _VP],
     "mdt_fx_plan_destroy": [_VP],
+    "mdt_fx_get_channel_exponents": [_VP, _VP],
+    "mdt_fx_get_split_scale": [_VP, ctypes.POINTER(_F)],
     "mdt_synth_i8": [ctypes.POINTER(SynthParams), _I, _I, _I, _VP, _VP],
     "mdt_fx_set_gains": [_VP, _VP, _VP],
     "mdt_calib_plan_create": [ctypes.POINTER(_VP), _I, _I],
@@ -276,9 +279,26 @@ class FxPlan:
         check(lib().mdt_fx_plan_create(ctypes.byref(self._h), NN, N, T, Ns, window, xprec))
         self.NN, self.N, self.T, self.Ns = NN, N, T, Ns
         self.nb = NN * (NN + 1) // 2
+        self.xprec = xprec
 
     def set_precision(self, xprec):
+        """X-engine precision; the X input's sample format and pitches follow
+        it, so re-allocate the X input and re-run the F-engine after a change."""
         check(lib().mdt_fx_set_precision(self._h, xprec))
+        self.xprec = xprec
+
+    def split_scale(self):
+        """power-of-two scale s of the FP16 sample formats (x*s is stored)"""
+        s = ctypes.c_float()
+        check(lib().mdt_fx_get_split_scale(self._h, ctypes.byref(s)))
+        return s.value
+
+    def channel_exponents(self):
+        """FP8 plans: int32 [N] exponents e_c of the last F-engine run; the
+        X-engine quantises e4m3(x*s*2^e_c)."""
+        e = torch.empty(self.N, dtype=torch.int32, device="cuda")
+        check(lib().mdt_fx_get_channel_exponents(self._h, _ptr(e, torch.int32)))
+        return e
 
     def set_gains(self, gains=None, delays=None):
         """Per-input complex gains (complex64 [NN]) and delays (float32 [NN],
@@ -312,10 +332,17 @@ class FxPlan:
         """X input buffer; returns the padded storage (contiguous, what the
         library is given).  Use xinput_view() for the [N][NN][Ns] view."""
         rp, cp = self.xinput_pitch()
+        if self.xprec == XPREC_FP8:  # 4-byte samples: FP16 (re, im)
+            return torch.empty((self.N, cp), dtype=torch.int32, device="cuda")
         return torch.empty((self.N, cp), dtype=torch.complex64, device="cuda")
 
     def xinput_view(self, buf):
+        """[N][NN][Ns] view: complex64 samples, or for FP8 plans float16
+        [N][NN][Ns][2] (re, im) of x*s."""
         rp, cp = self.xinput_pitch()
+        if self.xprec == XPREC_FP8:
+            return torch.as_strided(buf.view(torch.float16), (self.N, self.NN, self.Ns, 2),
+                                    (2 * cp, 2 * rp, 2, 1))
         return torch.as_strided(buf, (self.N, self.NN, self.Ns), (cp, rp, 1))
 
     def alloc_vis(self):
@@ -338,18 +365,18 @@ class FxPlan:
 
     def fengine(self, x8, xin):
         self._check_sizes(x8=x8, xin=xin)
-        check(lib().mdt_fx_fengine(self._h, _ptr(x8, torch.int8), _ptr(xin, torch.complex64)))
+        check(lib().mdt_fx_fengine(self._h, _ptr(x8, torch.int8), _ptr(xin)))
         return xin
 
     def xengine(self, xin, vis, acc64=None):
         self._check_sizes(xin=xin, vis=vis, acc64=acc64)
-        check(lib().mdt_fx_xengine(self._h, _ptr(xin, torch.complex64), _ptr(vis, torch.complex64),
+        check(lib().mdt_fx_xengine(self._h, _ptr(xin), _ptr(vis, torch.complex64),
                                    _ptr(acc64, torch.complex128) if acc64 is not None else None))
         return vis
 
     def dump(self, x8, xin, vis, acc64=None):
         self._check_sizes(x8=x8, xin=xin, vis=vis, acc64=acc64)
-        check(lib().mdt_fx_dump(self._h, _ptr(x8, torch.int8), _ptr(xin, torch.complex64),
+        check(lib().mdt_fx_dump(self._h, _ptr(x8, torch.int8), _ptr(xin),
                                 _ptr(vis, torch.complex64),
                                 _ptr(acc64, torch.complex128) if acc64 is not None else None))
         return vis
