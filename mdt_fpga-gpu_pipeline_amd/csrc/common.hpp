@@ -23,6 +23,12 @@ int check_launch(const char *what);
 
 inline bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// The dispatch packet carries each grid dimension in work-items as a 32-bit
+// value: a 1-D launch of `blocks` x `threads` must stay below 2^32 work-items
+// (beyond that the grid silently wraps).  Launches whose size follows the data
+// check this first.
+int check_grid(long blocks, long threads, const char *what);
+
 inline int ilog2(long v) {
   int l = 0;
   while ((1L << l) < v) l++;

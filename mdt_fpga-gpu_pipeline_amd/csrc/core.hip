@@ -50,6 +50,13 @@ int check_taps(int T) {
     return fail(MDT_ERR_INVALID_VALUE, "Ntaps=%d outside [%d, %d]", T, MDT_MIN_TAPS, MDT_MAX_TAPS);
   return MDT_SUCCESS;
 }
+int check_grid(long blocks, long threads, const char *what) {
+  if (blocks < 1 || threads < 1 || blocks > (0xFFFFFFFFL / threads))
+    return fail(MDT_ERR_UNSUPPORTED, "%s: %ld blocks x %ld threads exceed the 32-bit dispatch grid", what,
+                blocks, threads);
+  return MDT_SUCCESS;
+}
+
 int check_spectra(int Ns) {
   if (Ns < MDT_MIN_SPECTRA || Ns > MDT_MAX_SPECTRA)
     return fail(MDT_ERR_INVALID_VALUE, "Nspectra=%d outside [%d, %d]", Ns, MDT_MIN_SPECTRA,
@@ -181,9 +188,7 @@ __device__ __forceinline__ int8_t clip8(double v) {
   return (int8_t)(int)r;
 }
 
-__global__ void synth_kernel(mdt_synth_params p, int NN, int N, long total, int8_t *out) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
+__device__ __forceinline__ void synth_one(const mdt_synth_params &p, int NN, int N, long i, int8_t *out) {
   int n = (int)(i % N);
   long fe = i / N;
   int e = (int)(fe % NN);
@@ -212,6 +217,12 @@ __global__ void synth_kernel(mdt_synth_params p, int NN, int N, long total, int8
   }
   out[2 * i] = clip8(re);
   out[2 * i + 1] = clip8(im);
+}
+
+__global__ void synth_kernel(mdt_synth_params p, int NN, int N, long total, int8_t *out) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride)
+    synth_one(p, NN, N, i, out);
 }
 
 }  // namespace mdt
@@ -295,8 +306,10 @@ int mdt_synth_i8(const mdt_synth_params *p, int NpolsxNelements, int Nchannels, 
   if (p->ntones < 0 || p->ntones > 4 || p->rho < 0.0f || p->rho > 1.0f || p->max_delay < 0)
     return fail(MDT_ERR_INVALID_VALUE, "synth parameters out of range");
   long total = (long)nframes * NpolsxNelements * Nchannels;
-  hipLaunchKernelGGL(synth_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, *p,
-                     NpolsxNelements, Nchannels, total, out);
+  long blocks = (total + 255) / 256;
+  if (blocks > (1L << 20)) blocks = 1L << 20;  // grid-stride beyond 2^28 samples
+  hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, *p, NpolsxNelements,
+                     Nchannels, total, out);
   return check_launch("synth_kernel");
 }
 

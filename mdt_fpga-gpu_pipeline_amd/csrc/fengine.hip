@@ -445,7 +445,7 @@ static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   const long nsb = (a.Ns + C::SB - 1) / C::SB;
   const long neg = (a.NN + C::EB - 1) / C::EB;
   const long total = nsb * neg;
-  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "F-engine grid too large");
+  MDT_TRY(check_grid(total, C::THREADS, "F-engine"));
   dim3 grid((unsigned)total), block(C::THREADS);
   switch (a.T) {
     case 4: hipLaunchKernelGGL((fengine_kernel<N, 4>), grid, block, 0, st, a); break;

@@ -46,7 +46,7 @@ static int launch_fft(const float2 *in, float2 *out, long batch, const float2 *t
                       hipStream_t st) {
   using C = FftCfg<N>;
   long blocks = (batch + C::FPB - 1) / C::FPB;
-  if (blocks > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "batch too large");
+  MDT_TRY(check_grid(blocks, C::THREADS, "FFT batch"));
   if (inv)
     hipLaunchKernelGGL((fft_kernel<N, true>), dim3((unsigned)blocks), dim3(C::THREADS), 0, st, in,
                        out, batch, tw);

@@ -229,7 +229,7 @@ int xeng_launch(const XArgs &g0, int layout, int out, hipStream_t st) {
     g.nbJ = nbM;
   }
   long total = (long)g.tiles * g.batch;
-  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "X-engine grid too large");
+  MDT_TRY(check_grid(total, 256, "X-engine"));
   dim3 grid((unsigned)total), block(256);
   if (layout == XL_KM && out == XO_FULL)
     hipLaunchKernelGGL((xeng_f32_kernel<XL_KM, XO_FULL>), grid, block, 0, st, g);

@@ -216,7 +216,7 @@ int xeng_launch_f16split(const XArgs &g0, hipStream_t st) {
   const int nb = (g.m + HT - 1) / HT;
   g.tiles = nb * (nb + 1) / 2;
   long total = (long)g.tiles * g.batch;
-  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "X-engine grid too large");
+  MDT_TRY(check_grid(total, 256, "X-engine"));
   // power-of-two input scale (exact) chosen by the caller; products scale by sc^2
   const float sc = g.in_scale;
   if (!(sc > 0.f)) return fail(MDT_ERR_INVALID_VALUE, "FP16 split needs a positive input scale");

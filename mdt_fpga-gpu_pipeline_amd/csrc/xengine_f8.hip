@@ -232,7 +232,7 @@ int xeng_launch_f8(const XArgs &g0, const float *power, hipStream_t st) {
   const int nb = (g.m + Q8T - 1) / Q8T;
   g.tiles = nb * (nb + 1) / 2;
   long total = (long)g.tiles * g.batch;
-  if (total > 2147483647L) return fail(MDT_ERR_INVALID_VALUE, "X-engine grid too large");
+  MDT_TRY(check_grid(total, 256, "X-engine"));
   if (!(g.in_scale > 0.f)) return fail(MDT_ERR_INVALID_VALUE, "FP8 path needs a positive input scale");
   const float count = (float)g.m * (float)g.k;
   hipLaunchKernelGGL(xeng_f8_kernel, dim3((unsigned)total), dim3(256), 0, st, g, power, count);

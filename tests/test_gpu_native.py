@@ -139,6 +139,22 @@ def test_synth_deterministic_and_statistics():
     assert abs(rho - 0.3) < 0.02
 
 
+def test_synth_beyond_32bit_grid():
+    """the C4 input (8195 frames x 512 inputs x 1024 channels = 4.3e9 samples,
+    more than a 32-bit dispatch grid holds): every frame is generated, and
+    frame f equals a one-frame synthesis started at t0 = f N"""
+    NN, N, F = 512, 1024, 8195
+    assert F * NN * N > 2 ** 32
+    x = mdtfx.synth_i8(NN, N, F, seed=3, sigma=4.0, rho=0.3, max_delay=8, tones=((0.3, 12.0),))
+    for f in (0, 4096, F - 1):
+        one = mdtfx.synth_i8(NN, N, 1, seed=3, sigma=4.0, rho=0.3, max_delay=8, tones=((0.3, 12.0),), t0=f * N)
+        assert torch.equal(x[f], one[0]), f
+    rms = x[-64:].float().pow(2).mean().sqrt().item()
+    assert abs(rms - x[:64].float().pow(2).mean().sqrt().item()) < 0.05 * rms
+    del x
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (256, 4096, 8, 64), (130, 512, 8, 100), (3, 64, 4, 9)])
 def test_fp16_split_vs_fp64(NN, N, T, Ns):
     """FP16 hi+lo split X-engine within the 1e-5 autocorrelation-normalised bar
