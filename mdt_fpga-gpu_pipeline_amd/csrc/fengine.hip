@@ -19,14 +19,6 @@
 
 namespace mdt {
 
-// persistent-kernel configuration (spectra per item, transforms per exchange,
-// corner-turn method)
-constexpr int FE_SB = 4;
-constexpr int FE_NX = 2;
-constexpr int FE_WMODE = 0;
-constexpr bool FE_WPERSIST = true;
-constexpr int FE_MR = 1;
-
 template <int N>
 struct FCfg {
   static constexpr int P = fft_points(N);
@@ -174,271 +166,313 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   }
 }
 
-// Persistent variant for N in [512, 4096] (8 points per thread) and T in
-// {4, 8}: one workgroup per slot of the CU, looping over (input, spectrum
-// block) items.  Each thread keeps its T x 8 window taps in registers for the
-// whole kernel (the window is read from memory once per workgroup, not once per
-// spectrum), and the workgroup stages the per-pass twiddle table in LDS once.
-// Same arithmetic in the same order as fengine_kernel, so the output is
-// bit-identical.
-__device__ __forceinline__ float swap_pair(float v) {  // lane l <-> l ^ 1 (DPP quad_perm 1,0,3,2)
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+// Persistent kernel for N in [512, 4096] (8 points per thread, TPB = N/8
+// threads per transform) and T in {4, 8}.  A workgroup is Q sub-blocks of TPB
+// threads; an item is one input and 4Q consecutive spectra, sub-block q
+// taking spectra 4q..4q+3.  Each sub-block runs the FIR and the four
+// transforms with its own exchange buffer (the twiddle table is shared), then
+// the whole workgroup corner-turns through LDS so that every (channel, input)
+// row gets 4Q consecutive samples in one piece: 64 B and more per row instead
+// of 32 B, which is what the scattered-row writes of the channel-major output
+// need to run near the HBM write rate.  Per thread: 128-VGPR budget, taps read
+// per item from a lane-contiguous copy (one 16-byte load per 4 taps), the four
+// spectra transformed one at a time in place in the FIR accumulators.  The
+// arithmetic and its order are those of fengine_kernel and of the drop-in
+// mdt_pps_batch_i8 + mdt_fft_exec_c2c, so every path agrees bit for bit.
+template <int N, int SPLIT, bool GAIN>
+struct PCfg {
+  static constexpr int P = 8, TPB = N / P, SB = 4;
+  // sub-blocks per workgroup: up to 1024 threads (512 with the gain stage,
+  // whose phasors need the registers of two waves per SIMD)
+  static constexpr int QMAX = (GAIN ? 512 : 1024) / TPB;
+  static constexpr int Q = QMAX < 1 ? 1 : (QMAX < 8 ? QMAX : 8);
+  static constexpr int THREADS = Q * TPB;
+  static constexpr int SPAN = SB * Q;                 // spectra per item
+  static constexpr int ROWW = SPAN + 1;               // staging row (float2), padded against bank conflicts
+  static constexpr int FFT_LDS = lds_entries<N>();    // per sub-block
+  static constexpr int SPC = SPLIT == 2 ? 4 : 2;      // samples per 16-byte chunk of the output
+  static constexpr int CPR = SPAN / SPC;              // chunks per row
+  // channel rounds per staging pass: as many as fit in the exchange buffers
+  static constexpr int MR0 = (Q * FFT_LDS) / (TPB * ROWW);
+  static constexpr int MR = MR0 >= 4 ? 4 : (MR0 >= 2 ? 2 : 1);
+  static constexpr int STAGE = MR * TPB * ROWW;
+  static constexpr int BUF = Q * FFT_LDS > STAGE ? Q * FFT_LDS : STAGE;
+  static constexpr int TWN = pass_tw_entries<N, P>();
+};
+
+__global__ void arrange_taps_kernel(int N, int T, const float *__restrict__ W, float *__restrict__ Wt) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N * T) return;
+  const int TPB = N / 8, T4 = T / 4;
+  const int ii = idx & 3;
+  int r = idx >> 2;
+  const int t = r % TPB;
+  r /= TPB;
+  const int i4 = r % T4, m = r / T4;
+  Wt[idx] = W[(long)(4 * i4 + ii) * N + t + m * TPB];
 }
 
-// NX: transforms per LDS exchange round.  WMODE: corner turn through LDS
-// staging (0), lane-pair DPP swaps with SB = 4 (1), or each lane storing its
-// own SB-spectrum segment (2).  WPERSIST: taps stay in registers for the
-// whole kernel (true) or are reloaded (L2-resident) for each item and are dead
-// outside the FIR, which frees registers for larger SB.
-template <int N, int T, int SB, int NX, int WMODE, bool WPERSIST, int MR>
-__global__ __launch_bounds__(N / 8, 1) void fengine_persistent_kernel(FArgs a, long nitems) {
-  constexpr bool DPPW = (WMODE == 1);
-  static_assert(!DPPW || SB == 4, "the lane-pair corner turn writes 4 spectra per segment");
-  static_assert(SB % NX == 0, "spectra per block must be a multiple of NX");
-  constexpr int P = 8, TPB = N / P;
+bool persistent_shape(int N, int T) { return N >= 512 && N <= 4096 && (T == 4 || T == 8); }
+
+int arrange_taps(int N, int T, const float *W, float *Wt, hipStream_t st) {
+  if (!persistent_shape(N, T)) return fail(MDT_ERR_UNSUPPORTED, "no persistent F-engine for N=%d T=%d", N, T);
+  hipLaunchKernelGGL(arrange_taps_kernel, dim3((N * T + 255) / 256), dim3(256), 0, st, N, T, W, Wt);
+  return check_launch("arrange_taps_kernel");
+}
+
+// v as a value the compiler cannot see through: addresses derived from it
+// are computed where they are used instead of being hoisted (out of the item
+// loop, or to the top of the item) and held in registers across the phases
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// raw buffer resource word 3 for gfx9-family untyped buffer access (32-bit
+// data format, no swizzle); loads past num_records return zero
+constexpr int kBufferWord3 = 0x00020000;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// waves per SIMD the register budget targets: 4 (128 VGPRs) for the plain
+// outputs; the gain stage and the FP8 power sums at N >= 2048 need more room
+template <int N, int SPLIT, bool GAIN>
+constexpr int fe_waves_per_simd() {
+  return GAIN ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
+}
+
+template <int N, int T, int SPLIT, bool GAIN>
+__global__ __launch_bounds__((PCfg<N, SPLIT, GAIN>::THREADS), (fe_waves_per_simd<N, SPLIT, GAIN>())) void
+fengine_persistent_kernel(FArgs a, long nitems) {
+  using C = PCfg<N, SPLIT, GAIN>;
+  constexpr int P = C::P, TPB = C::TPB, SB = C::SB, Q = C::Q, SPAN = C::SPAN, ROWW = C::ROWW, MR = C::MR;
+  constexpr int SPC = C::SPC, CPR = C::CPR;
   constexpr int L = full_passes(N, P);
   constexpr int RL = N / ipow_c(P, L);
   constexpr int NPASS = L + (RL > 1 ? 1 : 0);
-  constexpr int TWN = pass_tw_entries<N, P>();
-  constexpr int ROW = SB + 1;
-  constexpr int FBUF = NX * lds_entries<N>();
-  constexpr int SBUF = MR * TPB * ROW;  // staging for MR channel rounds
-  constexpr int BUF = (WMODE != 0 || FBUF >= SBUF) ? FBUF : SBUF;
-  __shared__ float2 twl[TWN > 0 ? TWN : 1];
-  __shared__ float2 buf[BUF];
-  const int t = threadIdx.x;
+  __shared__ float2 twl[C::TWN];
+  __shared__ float2 buf[C::BUF];
+  const int tid = threadIdx.x;
+  // sub-block: uniform within a wave (TPB >= 64); made a scalar explicitly so
+  // that frame offsets and buffer bases stay in scalar registers
+  const int q = __builtin_amdgcn_readfirstlane(tid / TPB);
+  const int t = tid % TPB;
 
-  // per-pass twiddle table: entry [pass][jm][r] = W_{NS*R}^{jm*r}
-  for (int idx = t; idx < TWN; idx += TPB) {
+  // per-pass twiddle table: pass p >= 1 (stage NS = 8^p, radix R) holds
+  // W_{NS*R}^{jm*r} at [offset(p) + r*NS + jm]
+  for (int idx = tid; idx < C::TWN; idx += C::THREADS) {
     int pass = 1, off = 0, ns = P;
-    while (pass < NPASS - 1 && idx >= off + ns * 8) {
-      off += ns * 8;
+    while (pass < NPASS - 1 && idx >= off + ns * P) {
+      off += ns * P;
       ns *= P;
       pass++;
     }
     const int R = (pass < L) ? P : RL;
-    const int rel = idx - off, jm = rel / 8, r = rel % 8;
-    twl[idx] = (r < R) ? a.tw[jm * r * (N / (ns * R))] : make_float2(0.f, 0.f);
-  }
-  float w[T][P];
-  if constexpr (WPERSIST) {
-#pragma unroll
-    for (int i = 0; i < T; i++)
-#pragma unroll
-      for (int m = 0; m < P; m++) w[i][m] = a.W[(long)i * N + t + m * TPB];
+    const int rel = idx - off, r = rel / ns, jm = rel % ns;
+    twl[idx] = a.tw[jm * r * (N / (ns * R))];
   }
   __syncthreads();
-  const PassTw<P> tws{twl};
 
-  const int nsb = (a.Ns + SB - 1) / SB;
+  const int nsb = (a.Ns + SPAN - 1) / SPAN;
   const long G = gridDim.x;
   const long b = blockIdx.x;
   const long spec = (long)a.NN * N;
+  const __amdgpu_buffer_rsrc_t rtap =
+      __builtin_amdgcn_make_buffer_rsrc((void *)a.Wt, (short)0, T * N * 4, kBufferWord3);
+  float2 *fbuf = buf + q * C::FFT_LDS;  // this sub-block's exchange buffer
   float pw[P];  // split 2: this thread's channels' power over all its items
 #pragma unroll
   for (int m = 0; m < P; m++) pw[m] = 0.f;
   for (long base = 0; base < nitems; base += G) {
     // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
-    // spectrum blocks of one input) go to the same XCD in each round.
+    // spectrum blocks of one input: shared FIR frames, adjacent output pieces)
+    // go to the same XCD.
     const long logical = (G % 8 == 0) ? base + (b % 8) * (G / 8) + b / 8 : base + b;
     if (logical >= nitems) break;
     const int sb = (int)(logical % nsb);
     const int e = (int)(logical / nsb);
-    const int s0 = sb * SB;
-    const int scount = min(SB, a.Ns - s0);
-    const char2 *src = a.in + (long)s0 * spec + (long)e * N + t;
+    const int s0 = sb * SPAN;             // the item's first spectrum
+    const int sq = s0 + SB * q;           // this sub-block's first spectrum
+    const int scount = min(SB, a.Ns - sq);  // <= 0: a sub-block past Ns (garbage, not stored)
 
-    if constexpr (!WPERSIST) {
-#pragma unroll
-      for (int i = 0; i < T; i++)
-#pragma unroll
-        for (int m = 0; m < P; m++) w[i][m] = a.W[(long)i * N + t + m * TPB];
-    }
+    // FIR, one point at a time: its T taps, then its SB + T - 1 frames; frame
+    // f feeds spectrum sl through tap i = f - sl, and each output accumulates
+    // its taps in order i = 0..T-1.  Buffer loads: scalar item base and frame
+    // offsets, one 32-bit lane offset per point.  Frames past the input
+    // (partial last block) are clamped; their spectra are never stored.
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(a.in + (long)s0 * spec + (long)e * N), (short)0, 0x7fffffff, kBufferWord3);
+    const int fmax = a.Ns + a.T - 2 - s0;  // last frame of the input, relative to s0
+    int twoff = 0;
+    asm volatile("" : "+s"(twoff));
+    const PassTw<P> tws{twl + twoff};
+    int tl = opaque(t);
     float2 acc[SB][P];
 #pragma unroll
-    for (int sl = 0; sl < SB; sl++)
+    for (int m = 0; m < P; m++) {
+      float w[T];
 #pragma unroll
-      for (int m = 0; m < P; m++) acc[sl][m] = make_float2(0.f, 0.f);
+      for (int i4 = 0; i4 < T / 4; i4++) {
+        const v4u qv = __builtin_amdgcn_raw_buffer_load_b128(rtap, 16 * tl, 16 * ((m * (T / 4) + i4) * TPB), 0);
+        w[4 * i4] = __uint_as_float(qv[0]);
+        w[4 * i4 + 1] = __uint_as_float(qv[1]);
+        w[4 * i4 + 2] = __uint_as_float(qv[2]);
+        w[4 * i4 + 3] = __uint_as_float(qv[3]);
+      }
+      float2 x[SB + T - 1];
 #pragma unroll
-    for (int f = 0; f < SB + T - 1; f++) {
-      if (f < scount + T - 1) {
-        float2 x[P];
+      for (int f = 0; f < SB + T - 1; f++) {
+        const int fr = SB * q + f < fmax ? SB * q + f : fmax;
+        const unsigned short u =
+            __builtin_amdgcn_raw_buffer_load_b16(rin, 2 * (tl + m * TPB), (int)(2 * (long)fr * spec), 0);
+        x[f] = make_float2((float)(signed char)(u & 0xff), (float)(signed char)(u >> 8));
+      }
 #pragma unroll
-        for (int m = 0; m < P; m++) {
-          char2 v = src[(long)f * spec + m * TPB];
-          x[m] = make_float2((float)v.x, (float)v.y);
+      for (int sl = 0; sl < SB; sl++) {
+        float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i = 0; i < T; i++) {
+          y.x = __builtin_fmaf(w[i], x[sl + i].x, y.x);
+          y.y = __builtin_fmaf(w[i], x[sl + i].y, y.y);
         }
-#pragma unroll
-        for (int sl = 0; sl < SB; sl++) {
-          const int i = f - sl;
-          if (i >= 0 && i < T && sl < scount) {
-#pragma unroll
-            for (int m = 0; m < P; m++) {
-              acc[sl][m].x = __builtin_fmaf(w[i][m], x[m].x, acc[sl][m].x);
-              acc[sl][m].y = __builtin_fmaf(w[i][m], x[m].y, acc[sl][m].y);
-            }
-          }
-        }
+        acc[sl][m] = y;
       }
     }
-    // transforms in pairs (block-uniform trip count)
+    // transforms, one at a time, all SB of them: inside `if (sl < scount)`
+    // the FIR sums of the later spectra would be sunk into those branches,
+    // keeping every frame's samples live until the last transform
 #pragma unroll
-    for (int sp = 0; sp < SB; sp += NX) {
-      if (sp < scount) {
-        float2 (&pair)[NX][P] = *reinterpret_cast<float2(*)[NX][P]>(&acc[sp][0]);
-        fft_block_multi<N, P, false, NX>(pair, t, buf, tws);
+    for (int sl = 0; sl < SB; sl++) {
+      fft_block_tw<N, P, false>(acc[sl], opaque(t), fbuf, tws);
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (SPLIT == 2 && !GAIN) {
+        // branch-free: spectra past the input weigh zero (finite values
+        // times 0, so pw is unchanged bit for bit)
+        const float wgt = sl < scount ? a.split_scale : 0.f;
+#pragma unroll
+        for (int m = 0; m < P; m++) {
+          const float xr = acc[sl][m].x * wgt, xi = acc[sl][m].y * wgt;
+          pw[m] = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, pw[m]));
+        }
       }
     }
 
-    if (a.gain != nullptr) {
+    if constexpr (GAIN) {
+      tl = opaque(t);
       const float2 g = a.gain[e];
       const float tau = a.delay ? a.delay[e] : 0.0f;
 #pragma unroll
       for (int m = 0; m < P; m++) {
-        const int k = t + m * TPB;
+        const int k = tl + m * TPB;
         const float2 ph = cmul(g, delay_phasor(signed_bin(k, N), tau, N));
 #pragma unroll
         for (int sl = 0; sl < SB; sl++) acc[sl][m] = cmul(acc[sl][m], ph);
       }
-    }
-
-    if (a.split == 2) {
+      if constexpr (SPLIT == 2) {
 #pragma unroll
-      for (int m = 0; m < P; m++)
+        for (int m = 0; m < P; m++)
 #pragma unroll
-        for (int sl = 0; sl < SB; sl++)
-          if (sl < scount) {
-            const float xr = acc[sl][m].x * a.split_scale, xi = acc[sl][m].y * a.split_scale;
+          for (int sl = 0; sl < SB; sl++) {
+            const float wgt = sl < scount ? a.split_scale : 0.f;
+            const float xr = acc[sl][m].x * wgt, xi = acc[sl][m].y * wgt;
             pw[m] = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, pw[m]));
           }
+      }
     }
 
-    if constexpr (WMODE == 2) {
-      // each lane stores its own channels' SB-spectrum segments (16-byte
-      // stores), no LDS staging and no barriers
-      const bool full = (scount == SB) && ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
+    // corner turn through LDS, MR channel rounds per barrier pair: row
+    // (channel) r of a round holds the item's SPAN spectra; each 16-byte
+    // chunk of a row goes to one lane, the CPR chunks of a row to consecutive
+    // lanes, so each store instruction writes whole SPAN-sample row pieces
+    const bool vec = SPLIT == 2 ? (((a.rp | a.cp) & 3) == 0) : (((a.rp | a.cp) & 1) == 0);
+    const int scnt_item = min(SPAN, a.Ns - s0);
 #pragma unroll
-      for (int m = 0; m < P; m++) {
-        const int k = t + m * TPB;
-        float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0;
-        float2 o[SB];
+    for (int m0 = 0; m0 < P; m0 += MR) {
+      __syncthreads();
+      tl = opaque(t);
 #pragma unroll
-        for (int sl = 0; sl < SB; sl++) o[sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
-        if (full) {
+      for (int mr = 0; mr < MR; mr++)
 #pragma unroll
-          for (int sl = 0; sl < SB; sl += 2)
-            *(float4 *)(dst + sl) = make_float4(o[sl].x, o[sl].y, o[sl + 1].x, o[sl + 1].y);
-        } else {
+        for (int sl = 0; sl < SB; sl++)
+          buf[(mr * TPB + tl) * ROWW + SB * q + sl] =
+              (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
+      __syncthreads();
+      const int tid2 = opaque(tid);
 #pragma unroll
-          for (int sl = 0; sl < SB; sl++)
-            if (sl < scount) dst[sl] = o[sl];
-        }
-      }
-    } else if constexpr (DPPW) {
-      // corner turn without LDS: lanes 2g and 2g+1 hold channels k and k+1 for
-      // 4 spectra (two 16-byte chunks each); one DPP swap gives each lane pair
-      // the two halves of one channel's 32-byte segment for each of 2 stores.
-      const int odd = t & 1;
-      const bool full = (scount == SB) && ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
-#pragma unroll
-      for (int m = 0; m < P; m++) {
-        const int k = t + m * TPB;
-        float2 o[SB];
-#pragma unroll
-        for (int sl = 0; sl < SB; sl++) o[sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
-        if (full) {
-          const float4 c0 = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
-          const float4 c1 = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
-          const float4 snd = odd ? c0 : c1;
-          const float4 rcv =
-              make_float4(swap_pair(snd.x), swap_pair(snd.y), swap_pair(snd.z), swap_pair(snd.w));
-          float4 *d1 = (float4 *)(a.out + (long)(k - odd) * a.cp + (long)e * a.rp + s0 + 2 * odd);
-          float4 *d2 = (float4 *)(a.out + (long)(k + 1 - odd) * a.cp + (long)e * a.rp + s0 + 2 * odd);
-          *d1 = odd ? rcv : c0;
-          *d2 = odd ? c1 : rcv;
-        } else {
-          float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0;
-#pragma unroll
-          for (int sl = 0; sl < SB; sl++)
-            if (sl < scount) dst[sl] = o[sl];
-        }
-      }
-    } else {
-      // corner turn staged through LDS, MR channel rounds per barrier pair:
-      // each store instruction writes SB consecutive spectra of a (channel,
-      // input) row per SB/2 lanes
-      constexpr int CH = SB / 2;
-      const bool vec = ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
-#pragma unroll
-      for (int m0 = 0; m0 < P; m0 += MR) {
-        __syncthreads();
-#pragma unroll
-        for (int mr = 0; mr < MR; mr++)
-#pragma unroll
-          for (int sl = 0; sl < SB; sl++)
-            buf[mr * TPB * ROW + t * ROW + sl] =
-                (a.split == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
-        __syncthreads();
-#pragma unroll
-        for (int mr = 0; mr < MR; mr++)
-#pragma unroll
-          for (int v = 0; v < CH; v++) {
-            const int q = t + v * TPB;
-            const int tt = q / CH, pr = q % CH;
-            const int k = tt + (m0 + mr) * TPB;
-            const int sl = 2 * pr;
-            const long o = (long)k * a.cp + (long)e * a.rp + s0 + sl;
-            float2 *dst = a.out + o;
-            const float2 *src_ = buf + mr * TPB * ROW + tt * ROW;
-            const float2 v0 = src_[sl], v1 = src_[sl + 1];
-            if (a.split == 2) {  // 4-byte FP16 samples
-              unsigned *d4 = (unsigned *)a.out + o;
-              if (vec && sl + 1 < scount) {
-                *(uint2 *)d4 = make_uint2(half_sample(v0, a.split_scale), half_sample(v1, a.split_scale));
-              } else {
-                if (sl < scount) d4[0] = half_sample(v0, a.split_scale);
-                if (sl + 1 < scount) d4[1] = half_sample(v1, a.split_scale);
-              }
-            } else if (vec && sl + 1 < scount) {
-              *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
-            } else {
-              if (sl < scount) dst[0] = v0;
-              if (sl + 1 < scount) dst[1] = v1;
-            }
+      for (int v = 0; v < (MR * TPB * CPR) / C::THREADS; v++) {
+        const int c = tid2 + v * C::THREADS;
+        const int row = c / CPR, cc = c % CPR;
+        const int k = row % TPB + (m0 + row / TPB) * TPB;
+        const int s1 = cc * SPC;  // first spectrum of the chunk, relative to s0
+        const long o = (long)k * a.cp + (long)e * a.rp + s0 + s1;
+        const float2 *src_ = buf + row * ROWW + s1;
+        if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
+          unsigned *d4 = (unsigned *)a.out + o;
+          const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
+          const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
+          if (vec && s1 + 3 < scnt_item) {
+            *(uint4 *)d4 = make_uint4(h0, h1, h2, h3);
+          } else {
+            if (s1 < scnt_item) d4[0] = h0;
+            if (s1 + 1 < scnt_item) d4[1] = h1;
+            if (s1 + 2 < scnt_item) d4[2] = h2;
+            if (s1 + 3 < scnt_item) d4[3] = h3;
           }
+        } else {  // 8-byte samples, 2 per chunk
+          float2 *dst = a.out + o;
+          const float2 v0 = src_[0], v1 = src_[1];
+          if (vec && s1 + 1 < scnt_item) {
+            *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+          } else {
+            if (s1 < scnt_item) dst[0] = v0;
+            if (s1 + 1 < scnt_item) dst[1] = v1;
+          }
+        }
       }
-      __syncthreads();  // the staging buffer is the next item's FFT buffer
     }
+    __syncthreads();  // the staging buffer is the next item's exchange buffers
   }
-  if (a.split == 2) {
+  if constexpr (SPLIT == 2) {
 #pragma unroll
     for (int m = 0; m < P; m++) atomicAdd(&a.power[t + m * TPB], pw[m]);
   }
 }
 
-template <int N, int T, int SB, int NX, int WMODE, bool WPERSIST, int MR>
-static int launch_persistent(const FArgs &a, hipStream_t st) {
-  const long nitems = (long)a.NN * ((a.Ns + SB - 1) / SB);
+template <int N, int T, int SPLIT, bool GAIN>
+static int launch_persistent_k(const FArgs &a, hipStream_t st) {
+  using C = PCfg<N, SPLIT, GAIN>;
+  static_assert((C::MR * C::TPB * C::CPR) % C::THREADS == 0, "store chunks must divide among the threads");
+  const long nitems = (long)a.NN * ((a.Ns + C::SPAN - 1) / C::SPAN);
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SB, NX, WMODE, WPERSIST, MR>, N / 8,
-                                                   0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SPLIT, GAIN>,
+                                                   C::THREADS, 0) != hipSuccess)
     return fail(MDT_ERR_DEVICE, "device attributes for the F-engine grid");
   long grid = (long)cus * (per_cu > 0 ? per_cu : 1);
   if (grid > nitems) grid = nitems;
-  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SB, NX, WMODE, WPERSIST, MR>), dim3((unsigned)grid), dim3(N / 8), 0, st, a,
-                     nitems);
+  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SPLIT, GAIN>), dim3((unsigned)grid), dim3(C::THREADS), 0,
+                     st, a, nitems);
   return check_launch("fengine_persistent_kernel");
+}
+
+template <int N, int T>
+static int launch_persistent(const FArgs &a, hipStream_t st) {
+  if (a.Wt == nullptr) return fail(MDT_ERR_INVALID_VALUE, "persistent F-engine needs the arranged taps");
+  const bool gain = a.gain != nullptr;
+  switch (a.split) {
+    case 0: return gain ? launch_persistent_k<N, T, 0, true>(a, st) : launch_persistent_k<N, T, 0, false>(a, st);
+    case 1: return gain ? launch_persistent_k<N, T, 1, true>(a, st) : launch_persistent_k<N, T, 1, false>(a, st);
+    case 2: return gain ? launch_persistent_k<N, T, 2, true>(a, st) : launch_persistent_k<N, T, 2, false>(a, st);
+    default: return fail(MDT_ERR_INVALID_VALUE, "unknown F-engine output format %d", a.split);
+  }
 }
 
 template <int N>
 static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   using C = FCfg<N>;
   if constexpr (N >= 512 && N <= 4096) {
-    if (a.split == 2 && FE_WMODE != 0) return fail(MDT_ERR_UNSUPPORTED, "FP8 needs the LDS corner turn");
-    if (a.T == 4) return launch_persistent<N, 4, FE_SB, FE_NX, FE_WMODE, FE_WPERSIST, FE_MR>(a, st);
-    if (a.T == 8) return launch_persistent<N, 8, FE_SB, FE_NX, FE_WMODE, FE_WPERSIST, FE_MR>(a, st);
+    if (a.T == 4) return launch_persistent<N, 4>(a, st);
+    if (a.T == 8) return launch_persistent<N, 8>(a, st);
   }
   if (a.split == 2)
     return fail(MDT_ERR_UNSUPPORTED, "FP8 X-engine input needs Nchannels in [512, 4096] and Ntaps 4 or 8");

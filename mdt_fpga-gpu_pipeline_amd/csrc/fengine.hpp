@@ -8,6 +8,7 @@ namespace mdt {
 struct FArgs {
   const char2 *in;      // [Ns+T-1][NN][N] int8 (I, Q)
   const float *W;       // [T][N] FP32 window
+  const float *Wt;      // persistent kernel: the window as [m][i/4][t][4] (arrange_taps)
   const float2 *tw;     // W_N^k twiddles
   float2 *out;          // channel-major: (k, e, s) at k*cp + e*rp + s, complex FP32
   int NN, Ns, T;
@@ -41,5 +42,10 @@ __device__ __forceinline__ float2 split_sample(float2 v, float sc) {
 }
 
 int launch_fengine(const FArgs &a, int N, hipStream_t st);
+// shapes served by the persistent kernel (N in [512, 4096], T in {4, 8})
+bool persistent_shape(int N, int T);
+// the window [T][N] re-arranged for the persistent kernel: Wt[((m*(T/4) + i/4)*(N/8) + t)*4 + i%4]
+// = W[i*N + t + m*N/8]
+int arrange_taps(int N, int T, const float *W, float *Wt, hipStream_t st);
 
 }  // namespace mdt

@@ -104,9 +104,10 @@ constexpr int lds_entries() {
 }
 
 // Twiddle sources.  GlobalTw: the W_N^k table (global memory, L1/L2 cached).
-// PassTw: per-pass table in LDS, entry [pass][j mod NS][r] holding
-// W_{NS*R}^{(j mod NS)*r}, so one butterfly reads its R twiddles as R/2
-// contiguous 16-byte words (radix-8 passes only).
+// PassTw: per-pass table in LDS, entry [pass][r][j mod NS] holding
+// W_{NS*R}^{(j mod NS)*r}: for a given r, lanes with consecutive j read
+// consecutive words (no bank conflicts); radix-P passes hold P slots of NS
+// entries, the remainder pass R.
 struct GlobalTw {
   const float2 *__restrict__ tw;
   template <int N, int NS, int R>
@@ -118,16 +119,16 @@ template <int P>
 constexpr int pass_tw_offset(int pass) {  // entries before pass `pass` (pass >= 1)
   int off = 0, ns = P;
   for (int p = 1; p < pass; p++) {
-    off += ns * 8;
+    off += ns * P;
     ns *= P;
   }
   return off;
 }
 template <int N, int P>
-constexpr int pass_tw_entries() {
+constexpr int pass_tw_entries() {  // radix-P passes hold P slots of NS, the remainder pass RL
   constexpr int L = full_passes(N, P);
-  constexpr int NPASS = L + ((N / ipow_c(P, L)) > 1 ? 1 : 0);
-  return pass_tw_offset<P>(NPASS);
+  constexpr int RL = N / ipow_c(P, L);
+  return RL > 1 ? pass_tw_offset<P>(L) + ipow_c(P, L) * RL : pass_tw_offset<P>(L);
 }
 template <int P>
 struct PassTw {
@@ -135,9 +136,36 @@ struct PassTw {
   template <int N, int NS, int R>
   __device__ __forceinline__ float2 get(int jm, int r) const {
     constexpr int pass = ilog2_c(NS) / ilog2_c(P);
-    return tab[pass_tw_offset<P>(pass) + jm * 8 + r];
+    return tab[pass_tw_offset<P>(pass) + r * NS + jm];
   }
 };
+
+// Stockham scatter addresses as one base + compile-time offsets: with
+// lpad(i) = i + (i >> 4), lpad(base + r*NS) splits into wbase(j) + woff(r)
+// whenever the carries of (base + r*NS) >> 4 do not depend on j:
+//   NS % 16 == 0:    lpad(base) + r*NS*17/16
+//   NS == 1, R = 8:  8j + (j >> 1) + r        (8j + r, r < 8, never carries into bit 4 beyond j/2)
+//   NS == 1, R = 16: 17j + r
+//   NS == 8, R = 8:  (j/8)*68 + j%8 + 8r + (r >> 1)
+template <int NS, int R>
+constexpr bool split_scatter() {
+  return NS % 16 == 0 || (NS == 1 && (R == 8 || R == 16)) || (NS == 8 && R == 8);
+}
+template <int NS, int R>
+__device__ __forceinline__ int scatter_base(int j) {
+  const int jm = j & (NS - 1);
+  const int base = (j / NS) * NS * R + jm;
+  if constexpr (NS % 16 == 0) return lpad(base);
+  else if constexpr (NS == 1 && R == 8) return 8 * j + (j >> 1);
+  else if constexpr (NS == 1 && R == 16) return 17 * j;
+  else return (j / 8) * 68 + jm;  // NS == 8, R == 8
+}
+template <int NS, int R>
+constexpr int scatter_off(int r) {
+  if (NS % 16 == 0) return r * NS + (r * NS) / 16;
+  if (NS == 1) return r;
+  return 8 * r + (r >> 1);  // NS == 8, R == 8
+}
 
 template <int N, int P, int R, int NS, bool LAST, bool INV, typename TW>
 __device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
@@ -162,6 +190,10 @@ __device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds, con
     if constexpr (LAST) {
 #pragma unroll
       for (int r = 0; r < R; r++) x[q + r * Q] = v[r];
+    } else if constexpr (split_scatter<NS, R>()) {
+      float2 *wp = lds + scatter_base<NS, R>(j);
+#pragma unroll
+      for (int r = 0; r < R; r++) wp[scatter_off<NS, R>(r)] = v[r];
     } else {
       const int base = (j / NS) * NS * R + jm;
 #pragma unroll
@@ -180,8 +212,15 @@ __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, co
   constexpr int TPB = N / P;
   if constexpr (PASS > 0) {
     __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (TPB % 16 == 0) {  // lpad(t + m*TPB) = lpad(t) + m*TPB*17/16
+      const float2 *rp = lds + lpad(t);
 #pragma unroll
-    for (int m = 0; m < P; m++) x[m] = lds[lpad(t + m * TPB)];
+      for (int m = 0; m < P; m++) x[m] = rp[m * (TPB + TPB / 16)];
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; m++) x[m] = lds[lpad(t + m * TPB)];
+    }
     __syncthreads();
   }
   fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x, t, lds, tw);

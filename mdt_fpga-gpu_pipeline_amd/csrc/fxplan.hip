@@ -22,6 +22,7 @@ struct mdt_fx_plan_s {
   const float *delay = nullptr;
   float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15
   float *power = nullptr;  // FP8 path: per-channel power from the F-engine
+  float *Wt = nullptr;     // window arranged for the persistent F-engine (when it serves the shape)
   int row_pad = 8, chan_pad = 64;  // X input padding (samples) after each row / channel
   // pitches in samples; 16-byte aligned rows: even for 8-byte samples, a
   // multiple of 4 for the 4-byte FP8-path samples
@@ -106,6 +107,12 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
     hipLaunchKernelGGL(zero_kernel, dim3((Nchannels + 255) / 256), dim3(256), 0, 0, p->power, Nchannels);
     st = check_launch("zero_kernel");
   }
+  if (st == MDT_SUCCESS && persistent_shape(Nchannels, Ntaps)) {
+    if (hipMalloc(&p->Wt, sizeof(float) * (size_t)Ntaps * Nchannels) != hipSuccess)
+      st = fail(MDT_ERR_ALLOC, "arranged window");
+    else
+      st = arrange_taps(Nchannels, Ntaps, p->W, p->Wt, 0);
+  }
   if (st == MDT_SUCCESS) st = make_twiddles(Nchannels, &p->tw);
   if (st == MDT_SUCCESS && hipStreamSynchronize(0) != hipSuccess)
     st = fail(MDT_ERR_DEVICE, "plan table build");
@@ -113,6 +120,7 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
   if (st != MDT_SUCCESS) {
     (void)hipFree(p->W);
     (void)hipFree(p->power);
+    if (p->Wt) (void)hipFree(p->Wt);
     if (p->tw) (void)hipFree(p->tw);
     delete p;
     return st;
@@ -194,6 +202,7 @@ int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInput
   FArgs a{};
   a.in = (const char2 *)InSignal;
   a.W = plan->W;
+  a.Wt = plan->Wt;
   a.tw = plan->tw;
   a.out = (float2 *)XInputNative;
   a.NN = plan->NN;
@@ -265,6 +274,7 @@ int mdt_fx_plan_destroy(mdt_fx_plan plan) {
   if (!plan) return MDT_SUCCESS;
   if (plan->W) (void)hipFree(plan->W);
   if (plan->power) (void)hipFree(plan->power);
+  if (plan->Wt) (void)hipFree(plan->Wt);
   if (plan->tw) (void)hipFree(plan->tw);
   delete plan;
   return MDT_SUCCESS;
