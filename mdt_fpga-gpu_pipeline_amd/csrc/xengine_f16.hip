@@ -211,8 +211,165 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
     }
 }
 
+// ---------------------------------------------------------------------------
+// Whole-channel kernel for 128 < NN <= 256 (the C3 shape): one workgroup per
+// channel computes every 32 x 32 block of the lower triangle (36 blocks for
+// NN = 256), 12 waves x 3 blocks, so each sample is read from HBM exactly once
+// (128 x 128 tiles read a 256-input channel twice).  K chunks of 24 spectra
+// (3 MFMA K-steps) for all 256 rows are double-buffered in LDS: one barrier
+// per chunk, the next chunk's global loads in flight during the MFMAs.
+constexpr int WC_NB = 8;                  // 32-row blocks (NN <= 256)
+constexpr int WC_WAVES = 12, WC_THREADS = WC_WAVES * 64, WC_BPW = 3;  // 36 blocks
+constexpr int WC_KC = 24;                 // samples per K chunk
+constexpr int WC_ROWB = WC_KC * 4 + 16;   // bytes per row per plane (96 data + 16 pad)
+constexpr int WC_PLANE = WC_NB * 32 * WC_ROWB;
+constexpr int WC_BUF = 2 * WC_PLANE;      // hi + lo planes
+constexpr int WC_SEGS = WC_KC / 2;        // 16-byte segments (2 samples) per row per chunk
+constexpr int WC_LOADS = WC_NB * 32 * WC_SEGS / WC_THREADS;  // per thread per chunk
+static_assert(WC_NB * 32 * WC_SEGS % WC_THREADS == 0, "chunk loads must divide among the threads");
+
+// block list: (ra, rb), ra >= rb, ordered so that each wave's three blocks
+// mostly share their I rows
+__constant__ unsigned char kWcBlocks[WC_WAVES * WC_BPW][2] = {
+    {7, 0}, {7, 1}, {7, 2}, {7, 3}, {7, 4}, {7, 5}, {7, 6}, {7, 7}, {6, 0},
+    {6, 1}, {6, 2}, {6, 3}, {6, 4}, {6, 5}, {6, 6}, {5, 0}, {5, 1}, {5, 2},
+    {5, 3}, {5, 4}, {5, 5}, {4, 0}, {4, 1}, {4, 2}, {4, 3}, {4, 4}, {3, 0},
+    {3, 1}, {3, 2}, {3, 3}, {2, 0}, {2, 1}, {2, 2}, {1, 0}, {1, 1}, {0, 0}};
+
+struct WcPrefetch {
+  float4 v[WC_LOADS];
+  unsigned ok;  // bit 2i / 2i+1: first / second sample of load i valid
+};
+
+__device__ __forceinline__ void wc_fetch(WcPrefetch &p, const float2 *__restrict__ base, long ld, int NN,
+                                         int s0, int K) {
+  unsigned ok = 0;
+#pragma unroll
+  for (int i = 0; i < WC_LOADS; i++) {
+    const int c = threadIdx.x + i * WC_THREADS;
+    const int row = c / WC_SEGS, seg = c % WC_SEGS;
+    const int s = s0 + 2 * seg;
+    const bool r_ok = row < NN;
+    const int sc = s < K ? s : 0;
+    p.v[i] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + sc);
+    ok |= ((r_ok && s < K) ? 1u : 0u) << (2 * i);
+    ok |= ((r_ok && s + 1 < K) ? 1u : 0u) << (2 * i + 1);
+  }
+  p.ok = ok;
+}
+
+__device__ __forceinline__ void wc_stage(const WcPrefetch &p, char *lds) {
+#pragma unroll
+  for (int i = 0; i < WC_LOADS; i++) {
+    const int c = threadIdx.x + i * WC_THREADS;
+    const int row = c / WC_SEGS, seg = c % WC_SEGS;
+    const float4 v = p.v[i];
+    const unsigned m0 = (p.ok >> (2 * i)) & 1u ? 0xFFFFFFFFu : 0u;
+    const unsigned m1 = (p.ok >> (2 * i + 1)) & 1u ? 0xFFFFFFFFu : 0u;
+    *(uint2 *)(lds + row * WC_ROWB + seg * 8) = make_uint2(__float_as_uint(v.x) & m0, __float_as_uint(v.z) & m1);
+    *(uint2 *)(lds + WC_PLANE + row * WC_ROWB + seg * 8) =
+        make_uint2(__float_as_uint(v.y) & m0, __float_as_uint(v.w) & m1);
+  }
+}
+
+__global__ __launch_bounds__(WC_THREADS, 3) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
+  __shared__ __attribute__((aligned(16))) char lds[2 * WC_BUF];
+
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const int batch = (int)((nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w);
+  const float2 *base = g.Abase + (long)batch * g.Abatch;
+  const int NN = g.m, K = g.k;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l32 = lane & 31, kh = lane >> 5;
+
+  int ra[WC_BPW], rb[WC_BPW];
+  bool act[WC_BPW];
+#pragma unroll
+  for (int b = 0; b < WC_BPW; b++) {
+    ra[b] = kWcBlocks[wave * WC_BPW + b][0];
+    rb[b] = kWcBlocks[wave * WC_BPW + b][1];
+    act[b] = ra[b] * 32 < NN;  // rb <= ra
+  }
+  f32x16 accR[WC_BPW], accI[WC_BPW];
+#pragma unroll
+  for (int b = 0; b < WC_BPW; b++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      accR[b][r] = 0.f;
+      accI[b][r] = 0.f;
+    }
+
+  WcPrefetch pf;
+  wc_fetch(pf, base, g.ldI, NN, 0, K);
+  wc_stage(pf, lds);
+  int cur = 0;
+  for (int s0 = 0; s0 < K; s0 += WC_KC) {
+    __syncthreads();  // chunk `cur` staged; the other buffer is free
+    const bool more = s0 + WC_KC < K;
+    if (more) wc_fetch(pf, base, g.ldI, NN, s0 + WC_KC, K);
+    const char *bufp = lds + cur * WC_BUF;
+#pragma unroll
+    for (int ks = 0; ks < WC_KC / 8; ks++) {
+      const int koff = ks * 32 + kh * 16;
+      // per block: its operands, then its six MFMAs (the three split
+      // products, real and imaginary parts)
+#pragma unroll
+      for (int b = 0; b < WC_BPW; b++) {
+        if (!act[b]) continue;
+        const int irow = ra[b] * 32 + l32, jrow = rb[b] * 32 + l32;
+        const half8 Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
+        const half8 Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
+        const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
+        const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
+        const half8 Ph = rot_conj(Ih), Pl = rot_conj(Il);
+        accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jh, accR[b], 0, 0, 0);
+        accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jh, accI[b], 0, 0, 0);
+        accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jl, accR[b], 0, 0, 0);
+        accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jl, accI[b], 0, 0, 0);
+        accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il, Jh, accR[b], 0, 0, 0);
+        accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl, Jh, accI[b], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (more) wc_stage(pf, lds + (cur ^ 1) * WC_BUF);
+    cur ^= 1;
+  }
+
+  double2 *acc = (double2 *)g.acc64;
+#pragma unroll
+  for (int b = 0; b < WC_BPW; b++) {
+    if (!act[b]) continue;
+    const int bcol = rb[b] * 32 + l32;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int arow = ra[b] * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+      if (arow < NN && bcol < NN && arow >= bcol) {
+        const long o = (long)batch * g.Cbatch + tri_bl16(arow, bcol);
+        const float vr = oscale * accR[b][r], vi = -oscale * accI[b][r];
+        g.Cbase[o] = make_float2(vr, vi);
+        if (acc) {
+          double2 x = acc[o];
+          x.x += (double)vr;
+          x.y += (double)vi;
+          acc[o] = x;
+        }
+      }
+    }
+  }
+}
+
 int xeng_launch_f16split(const XArgs &g0, hipStream_t st) {
   XArgs g = g0;
+  if (g.m > 128 && g.m <= WC_NB * 32 && g.ldI == g.ldJ && g.Abase == g.Bbase) {
+    MDT_TRY(check_grid(g.batch, WC_THREADS, "X-engine"));
+    const float sc = g.in_scale;
+    if (!(sc > 0.f)) return fail(MDT_ERR_INVALID_VALUE, "FP16 split needs a positive input scale");
+    hipLaunchKernelGGL(xeng_f16split_chan_kernel, dim3((unsigned)g.batch), dim3(WC_THREADS), 0, st, g,
+                       g.alpha.x / (sc * sc));
+    return check_launch("xeng_f16split_chan_kernel");
+  }
   const int nb = (g.m + HT - 1) / HT;
   g.tiles = nb * (nb + 1) / 2;
   long total = (long)g.tiles * g.batch;
