@@ -214,36 +214,40 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
 // ---------------------------------------------------------------------------
 // Whole-channel kernel for 128 < NN <= 256 (the C3 shape): one workgroup per
 // channel computes every 32 x 32 block of the lower triangle (36 blocks for
-// NN = 256), 12 waves x 3 blocks, so each sample is read from HBM exactly once
-// (128 x 128 tiles read a 256-input channel twice).  K chunks of 24 spectra
-// (3 MFMA K-steps) for all 256 rows are double-buffered in LDS: one barrier
-// per chunk, the next chunk's global loads in flight during the MFMAs.
+// NN = 256), so each sample is read from HBM exactly once (128 x 128 tiles
+// read a 256-input channel twice).  8 waves, two per SIMD with 256 registers
+// each: waves 0-3 own 5 blocks and waves 4-7 own 4, so each SIMD carries 9.
+// K chunks of 16 spectra for all 256 rows are double-buffered in LDS (one
+// barrier per chunk) and prefetched two chunks ahead in registers, so about
+// 64 KB of loads per CU are in flight while the MFMAs run.  Consecutive blocks
+// of a wave mostly share their I rows; the I operand is reloaded only when
+// the row changes.
 constexpr int WC_NB = 8;                  // 32-row blocks (NN <= 256)
-constexpr int WC_WAVES = 12, WC_THREADS = WC_WAVES * 64, WC_BPW = 3;  // 36 blocks
-constexpr int WC_KC = 24;                 // samples per K chunk
-constexpr int WC_ROWB = WC_KC * 4 + 16;   // bytes per row per plane (96 data + 16 pad)
+constexpr int WC_WAVES = 8, WC_THREADS = WC_WAVES * 64, WC_BPW = 5;
+constexpr int WC_KC = 16;                 // samples per K chunk
+constexpr int WC_ROWB = WC_KC * 4 + 16;   // bytes per row per plane (64 data + 16 pad)
 constexpr int WC_PLANE = WC_NB * 32 * WC_ROWB;
 constexpr int WC_BUF = 2 * WC_PLANE;      // hi + lo planes
 constexpr int WC_SEGS = WC_KC / 2;        // 16-byte segments (2 samples) per row per chunk
 constexpr int WC_LOADS = WC_NB * 32 * WC_SEGS / WC_THREADS;  // per thread per chunk
 static_assert(WC_NB * 32 * WC_SEGS % WC_THREADS == 0, "chunk loads must divide among the threads");
 
-// block list: (ra, rb), ra >= rb, ordered so that each wave's three blocks
-// mostly share their I rows
+// block list (ra, rb), ra >= rb, 5 per wave for waves 0-3 and 4 (+ an unused
+// slot, ra = 255) for waves 4-7
 __constant__ unsigned char kWcBlocks[WC_WAVES * WC_BPW][2] = {
-    {7, 0}, {7, 1}, {7, 2}, {7, 3}, {7, 4}, {7, 5}, {7, 6}, {7, 7}, {6, 0},
-    {6, 1}, {6, 2}, {6, 3}, {6, 4}, {6, 5}, {6, 6}, {5, 0}, {5, 1}, {5, 2},
-    {5, 3}, {5, 4}, {5, 5}, {4, 0}, {4, 1}, {4, 2}, {4, 3}, {4, 4}, {3, 0},
-    {3, 1}, {3, 2}, {3, 3}, {2, 0}, {2, 1}, {2, 2}, {1, 0}, {1, 1}, {0, 0}};
+    {7, 0}, {7, 1}, {7, 2}, {7, 3}, {7, 4},   {7, 5}, {7, 6}, {7, 7}, {6, 0}, {6, 1},
+    {6, 2}, {6, 3}, {6, 4}, {6, 5}, {6, 6},   {5, 0}, {5, 1}, {5, 2}, {5, 3}, {5, 4},
+    {5, 5}, {4, 0}, {4, 1}, {4, 2}, {255, 0}, {4, 3}, {4, 4}, {3, 0}, {3, 1}, {255, 0},
+    {3, 2}, {3, 3}, {2, 0}, {2, 1}, {255, 0}, {2, 2}, {1, 0}, {1, 1}, {0, 0}, {255, 0}};
 
 struct WcPrefetch {
   float4 v[WC_LOADS];
-  unsigned ok;  // bit 2i / 2i+1: first / second sample of load i valid
+  unsigned ok0, ok1;  // bit i: first / second sample of load i valid
 };
 
 __device__ __forceinline__ void wc_fetch(WcPrefetch &p, const float2 *__restrict__ base, long ld, int NN,
                                          int s0, int K) {
-  unsigned ok = 0;
+  unsigned ok0 = 0, ok1 = 0;
 #pragma unroll
   for (int i = 0; i < WC_LOADS; i++) {
     const int c = threadIdx.x + i * WC_THREADS;
@@ -252,10 +256,11 @@ __device__ __forceinline__ void wc_fetch(WcPrefetch &p, const float2 *__restrict
     const bool r_ok = row < NN;
     const int sc = s < K ? s : 0;
     p.v[i] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + sc);
-    ok |= ((r_ok && s < K) ? 1u : 0u) << (2 * i);
-    ok |= ((r_ok && s + 1 < K) ? 1u : 0u) << (2 * i + 1);
+    ok0 |= ((r_ok && s < K) ? 1u : 0u) << i;
+    ok1 |= ((r_ok && s + 1 < K) ? 1u : 0u) << i;
   }
-  p.ok = ok;
+  p.ok0 = ok0;
+  p.ok1 = ok1;
 }
 
 __device__ __forceinline__ void wc_stage(const WcPrefetch &p, char *lds) {
@@ -264,15 +269,49 @@ __device__ __forceinline__ void wc_stage(const WcPrefetch &p, char *lds) {
     const int c = threadIdx.x + i * WC_THREADS;
     const int row = c / WC_SEGS, seg = c % WC_SEGS;
     const float4 v = p.v[i];
-    const unsigned m0 = (p.ok >> (2 * i)) & 1u ? 0xFFFFFFFFu : 0u;
-    const unsigned m1 = (p.ok >> (2 * i + 1)) & 1u ? 0xFFFFFFFFu : 0u;
+    const unsigned m0 = (p.ok0 >> i) & 1u ? 0xFFFFFFFFu : 0u;
+    const unsigned m1 = (p.ok1 >> i) & 1u ? 0xFFFFFFFFu : 0u;
     *(uint2 *)(lds + row * WC_ROWB + seg * 8) = make_uint2(__float_as_uint(v.x) & m0, __float_as_uint(v.z) & m1);
     *(uint2 *)(lds + WC_PLANE + row * WC_ROWB + seg * 8) =
         make_uint2(__float_as_uint(v.y) & m0, __float_as_uint(v.w) & m1);
   }
 }
 
-__global__ __launch_bounds__(WC_THREADS, 3) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
+// the MFMAs of one chunk (2 K-steps) over the wave's blocks
+__device__ __forceinline__ void wc_chunk(const char *bufp, const int (&ra)[WC_BPW], const int (&rb)[WC_BPW],
+                                         const bool (&act)[WC_BPW], int l32, int kh, f32x16 (&accR)[WC_BPW],
+                                         f32x16 (&accI)[WC_BPW]) {
+#pragma unroll
+  for (int ks = 0; ks < WC_KC / 8; ks++) {
+    const int koff = ks * 32 + kh * 16;
+    half8 Ih, Il, Ph, Pl;
+    int cached = -1;
+#pragma unroll
+    for (int b = 0; b < WC_BPW; b++) {
+      if (!act[b]) continue;
+      if (ra[b] != cached) {  // uniform: consecutive blocks mostly share the I row
+        const int irow = ra[b] * 32 + l32;
+        Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
+        Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
+        Ph = rot_conj(Ih);
+        Pl = rot_conj(Il);
+        cached = ra[b];
+      }
+      const int jrow = rb[b] * 32 + l32;
+      const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
+      const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
+      accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jh, accR[b], 0, 0, 0);
+      accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jh, accI[b], 0, 0, 0);
+      accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jl, accR[b], 0, 0, 0);
+      accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jl, accI[b], 0, 0, 0);
+      accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il, Jh, accR[b], 0, 0, 0);
+      accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl, Jh, accI[b], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
   __shared__ __attribute__((aligned(16))) char lds[2 * WC_BUF];
 
   const long nwg = gridDim.x;
@@ -290,7 +329,7 @@ __global__ __launch_bounds__(WC_THREADS, 3) void xeng_f16split_chan_kernel(XArgs
   for (int b = 0; b < WC_BPW; b++) {
     ra[b] = kWcBlocks[wave * WC_BPW + b][0];
     rb[b] = kWcBlocks[wave * WC_BPW + b][1];
-    act[b] = ra[b] * 32 < NN;  // rb <= ra
+    act[b] = ra[b] * 32 < NN;  // rb <= ra; the unused slot has ra = 255
   }
   f32x16 accR[WC_BPW], accI[WC_BPW];
 #pragma unroll
@@ -301,40 +340,25 @@ __global__ __launch_bounds__(WC_THREADS, 3) void xeng_f16split_chan_kernel(XArgs
       accI[b][r] = 0.f;
     }
 
-  WcPrefetch pf;
-  wc_fetch(pf, base, g.ldI, NN, 0, K);
-  wc_stage(pf, lds);
-  int cur = 0;
-  for (int s0 = 0; s0 < K; s0 += WC_KC) {
-    __syncthreads();  // chunk `cur` staged; the other buffer is free
-    const bool more = s0 + WC_KC < K;
-    if (more) wc_fetch(pf, base, g.ldI, NN, s0 + WC_KC, K);
-    const char *bufp = lds + cur * WC_BUF;
-#pragma unroll
-    for (int ks = 0; ks < WC_KC / 8; ks++) {
-      const int koff = ks * 32 + kh * 16;
-      // per block: its operands, then its six MFMAs (the three split
-      // products, real and imaginary parts)
-#pragma unroll
-      for (int b = 0; b < WC_BPW; b++) {
-        if (!act[b]) continue;
-        const int irow = ra[b] * 32 + l32, jrow = rb[b] * 32 + l32;
-        const half8 Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
-        const half8 Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
-        const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
-        const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
-        const half8 Ph = rot_conj(Ih), Pl = rot_conj(Il);
-        accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jh, accR[b], 0, 0, 0);
-        accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jh, accI[b], 0, 0, 0);
-        accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jl, accR[b], 0, 0, 0);
-        accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jl, accI[b], 0, 0, 0);
-        accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il, Jh, accR[b], 0, 0, 0);
-        accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl, Jh, accI[b], 0, 0, 0);
-      }
-      __builtin_amdgcn_sched_barrier(0);
+  // two-deep register prefetch: pa holds chunk c+1, pb chunk c+2 while chunk
+  // c is multiplied out of LDS
+  const int nchunk = (K + WC_KC - 1) / WC_KC;
+  WcPrefetch pa, pb;
+  wc_fetch(pa, base, g.ldI, NN, 0, K);
+  wc_stage(pa, lds);
+  if (nchunk > 1) wc_fetch(pa, base, g.ldI, NN, WC_KC, K);
+  if (nchunk > 2) wc_fetch(pb, base, g.ldI, NN, 2 * WC_KC, K);
+  for (int c = 0; c < nchunk; c += 2) {
+    __syncthreads();  // chunk c staged in buffer 0; buffer 1 is free
+    wc_chunk(lds, ra, rb, act, l32, kh, accR, accI);
+    if (c + 1 < nchunk) wc_stage(pa, lds + WC_BUF);
+    if (c + 3 < nchunk) wc_fetch(pa, base, g.ldI, NN, (c + 3) * WC_KC, K);
+    if (c + 1 < nchunk) {
+      __syncthreads();  // chunk c+1 staged in buffer 1; buffer 0 is free
+      wc_chunk(lds + WC_BUF, ra, rb, act, l32, kh, accR, accI);
+      if (c + 2 < nchunk) wc_stage(pb, lds);
+      if (c + 4 < nchunk) wc_fetch(pb, base, g.ldI, NN, (c + 4) * WC_KC, K);
     }
-    if (more) wc_stage(pf, lds + (cur ^ 1) * WC_BUF);
-    cur ^= 1;
   }
 
   double2 *acc = (double2 *)g.acc64;
@@ -357,6 +381,7 @@ __global__ __launch_bounds__(WC_THREADS, 3) void xeng_f16split_chan_kernel(XArgs
         }
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
