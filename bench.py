@@ -243,11 +243,11 @@ def main():
     traffic = load_traffic()
     if f16_ms >= x16_ms:
         ach = f_bytes / (f16_ms * 1e-3) / 1e9
-        roof = {"kernel": "fengine_kernel<4096,8>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+        roof = {"kernel": "fengine_persistent_kernel<4096, 8, 1, false>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get("fengine")}
     else:
         ach = x_flops / (x16_ms * 1e-3) / 1e12
-        roof = {"kernel": "xeng_f16split_kernel", "bound": "mfma", "achieved": ach, "peak": FP16_PEAK_TFLOPS,
+        roof = {"kernel": "xeng_f16split_chan_kernel", "bound": "mfma", "achieved": ach, "peak": FP16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP16_PEAK_TFLOPS, "traffic": traffic.get("xengine_fp16split")}
 
     out = {
