@@ -156,6 +156,23 @@ int mdt_history_shift(void *FInput, int NpolsxNelements, int Nchannels, int Ntap
  * (including its mapping of 0x80000000 to +4096.0).  n words -> n floats. */
 int mdt_unpack_u32(const uint32_t *words, float *out, size_t n, hipStream_t stream);
 
+/* ---------------------------------------------------------------- depacketiser
+ * Replaces the client's rank-1 loop (MDT_OneStation_client.cu:846-911 for the
+ * first block, first_block = 1; 945-1005 in steady state, first_block = 0) on
+ * the device.  npkts packets of words_per_pkt uint32 words each: station,
+ * SNAP, packet counter, unused, then SperElement x InputsPerSNAP[snap]
+ * (re, im) Fix_32_19 word pairs.  Each sample goes to
+ * Stacks[stack][spectrum][element][channel] (2 x Nspectra x NpolsxNelements x
+ * Nchannels complex64, the reference's h_Stacks), decoded as mdt_unpack_u32.
+ * counts (device, 3 x uint64) accumulates the samples written to stacks 0 and
+ * 1 (the reference's Nsamples[]) and the samples dropped because their header
+ * or index would land outside the stacks (the reference writes those out of
+ * bounds).  Two packets that address the same sample race, as in any
+ * parallel order. */
+int mdt_depacketise(const uint32_t *pkts, int npkts, int words_per_pkt, int SperElement,
+                    int NpolsxNelements, int Nchannels, int Nspectra, int Ntaps, int first_block,
+                    mdt_complex *Stacks, unsigned long long *counts, hipStream_t stream);
+
 /* ---------------------------------------------------------------- native fused path
  * One dump: int8 (I,Q) samples [Ns+T-1][NN][N][2] -> fused PFB FIR + FFT +
  * corner turn -> channel-major X input [N][NN][Ns] complex FP32 (row and

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "wire.hpp"
 #include "fft_core.hpp"
 
 namespace mdt {
@@ -148,13 +149,6 @@ __global__ __launch_bounds__(256) void reorder_kernel(int NN, int N, int Ns,
 // else 2^-19 * (int)w, product in double, stored as float.  (int)(2^31) wraps
 // to INT_MIN, so 0x80000000 -> +4096.0.  Here: the wrapped int32 value v,
 // negated for w >= 2^31, times 2^-19 in double, rounded to float.
-__device__ __forceinline__ float unpack_word(uint32_t w) {
-  if (w >= 2147483648u) {
-    int v = (int)(uint32_t)(0u - w);  // 2^32 - w, modulo 2^32
-    return (float)(-1.9073486328125e-06 * (double)v);
-  }
-  return (float)(1.9073486328125e-06 * (double)(int)w);
-}
 
 __global__ void unpack_kernel(const uint32_t *__restrict__ w, float *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -244,7 +244,7 @@ template <int N, int T, int SPLIT, bool GAIN>
 __global__ __launch_bounds__((PCfg<N, SPLIT, GAIN>::THREADS), (fe_waves_per_simd<N, SPLIT, GAIN>())) void
 fengine_persistent_kernel(FArgs a, long nitems) {
   using C = PCfg<N, SPLIT, GAIN>;
-  constexpr int P = C::P, TPB = C::TPB, SB = C::SB, Q = C::Q, SPAN = C::SPAN, ROWW = C::ROWW, MR = C::MR;
+  constexpr int P = C::P, TPB = C::TPB, SB = C::SB, SPAN = C::SPAN, ROWW = C::ROWW, MR = C::MR;
   constexpr int SPC = C::SPC, CPR = C::CPR;
   constexpr int L = full_passes(N, P);
   constexpr int RL = N / ipow_c(P, L);

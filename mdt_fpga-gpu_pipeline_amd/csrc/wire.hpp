@@ -1,0 +1,28 @@
+// wire.hpp -- the SNAP wire format shared by the unpack and depacketiser
+// kernels.
+#pragma once
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace mdt {
+
+// One Fix_32_19 word -> float, MDT_OneStation_client.cu:898-907:
+// w >= 2^31 -> -2^-19 * (int)(2^32 - w), else 2^-19 * (int)w, the product in
+// double, stored as float.  (int)(2^31) wraps to INT_MIN, so 0x80000000 maps
+// to +4096.0 exactly as in the reference.
+__device__ __forceinline__ float unpack_word(uint32_t w) {
+  if (w >= 2147483648u) {
+    int v = (int)(uint32_t)(0u - w);  // 2^32 - w, modulo 2^32
+    return (float)(-1.9073486328125e-06 * (double)v);
+  }
+  return (float)(1.9073486328125e-06 * (double)(int)w);
+}
+
+// station / SNAP geometry, MDT_OneStation_client.h:64-68
+constexpr int kInputsPerStation = 38;
+constexpr int kSnapsPerStation = 4;
+__device__ constexpr int kInputsPerSnap[kSnapsPerStation] = {10, 10, 10, 8};
+
+}  // namespace mdt

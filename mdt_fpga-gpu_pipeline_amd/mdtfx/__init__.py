@@ -84,6 +84,7 @@ _SIGS = {
     "mdt_xengine_herk": [_VP, _I, _I, _F, _VP, _VP, _I],
     "mdt_history_shift": [_VP, _I, _I, _I, _I, _I, _VP],
     "mdt_unpack_u32": [_VP, _VP, _SZ, _VP],
+    "mdt_depacketise": [_VP, _I, _I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP],
     "mdt_fx_plan_create": [ctypes.POINTER(_VP), _I, _I, _I, _I, _I, _I],
     "mdt_fx_set_stream": [_VP, _VP],
     "mdt_fx_set_precision": [_VP, _I],
@@ -391,6 +392,25 @@ class FxPlan:
             self.destroy()
         except Exception:
             pass
+
+
+def depacketise(pkts, SperElement, NN, N, Ns, T, first_block, stacks=None, counts=None):
+    """SNAP packets (int32/uint32 tensor [npkts][words] on the GPU) -> the
+    F-input stacks [2][Ns][NN][N] complex64; counts (uint64 as int64 [3]):
+    samples to stack 0, to stack 1, dropped.  Accumulates into the given
+    buffers (new zeroed ones when None)."""
+    if pkts.dim() != 2 or pkts.element_size() != 4:
+        raise ValueError("packets must be a 2-D tensor of 32-bit words")
+    if stacks is None:
+        stacks = torch.zeros((2, Ns, NN, N), dtype=torch.complex64, device="cuda")
+    if counts is None:
+        counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+    if stacks.numel() < 2 * Ns * NN * N or counts.numel() < 3:
+        raise ValueError("stacks or counts too small")
+    check(lib().mdt_depacketise(_ptr(pkts), pkts.shape[0], pkts.shape[1], SperElement, NN, N, Ns, T,
+                                1 if first_block else 0, _ptr(stacks, torch.complex64),
+                                _ptr(counts, torch.int64), _stream()))
+    return stacks, counts
 
 
 def synth_i8(NN, N, nframes, seed=1, sigma=16.0, rho=0.0, max_delay=0, tones=(), t0=0, out=None):

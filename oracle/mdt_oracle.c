@@ -434,7 +434,11 @@ int orc_count_unpack_mismatch_all(const float *unused, long *first_bad) {
  * word 3 unused, then SperElement x InputsPerSNAP[snap] (re, im) word pairs
  * (client.cu:888-891).  Element offset = station*38 + sum of inputs of lower
  * SNAP indices + j (client.cu:877-885; client.h:64-68).  stacks holds
- * 2 * Ns * NN * N complex floats; nsamples[2] counts samples per stack. */
+ * 2 * Ns * NN * N complex floats; nsamples[0..1] counts samples per stack.
+ * Where the reference would write outside its stacks (SNAP >= 4, elements
+ * beyond NN, a payload longer than the packet, a history longer than a stack,
+ * a steady-state index before the first sample) the sample is skipped and
+ * counted in nsamples[2], the rule the device kernel implements. */
 static const int kInputsPerSnap[4] = {10, 10, 10, 8};
 
 void orc_depacketise(const uint32_t *pkts, int npkts, int words_per_pkt, int SperElement,
@@ -445,8 +449,18 @@ void orc_depacketise(const uint32_t *pkts, int npkts, int words_per_pkt, int Spe
   for (int p = 0; p < npkts; p++) {
     const uint32_t *buf = pkts + (long)p * words_per_pkt;
     uint32_t station = buf[0], snap = buf[1], cnt = buf[2];
+    int hdr = snap < 4 && station < 65536u;
+    long elem0 = 0;
+    int nin = 0;
+    if (hdr) {
+      for (uint32_t k = 0; k < snap; k++) elem0 += kInputsPerSnap[k];
+      elem0 += (long)station * 38;
+      nin = kInputsPerSnap[snap];
+      hdr = elem0 + nin <= NN && 4 + 2L * SperElement * nin <= words_per_pkt;
+    }
     for (int i = 0; i < SperElement; i++) {
       long sample, stack, spec;
+      int ok = hdr;
       if (first_block) {
         sample = (long)cnt * SperElement + i;
         if (sample < begins) {
@@ -456,17 +470,20 @@ void orc_depacketise(const uint32_t *pkts, int npkts, int words_per_pkt, int Spe
           stack = 0;
         }
         spec = sample / N;
+        ok = ok && spec < Ns;
       } else {
         sample = (long)cnt * SperElement - begins + i;
-        stack = (sample / per_stack) % 2;
-        spec = (sample / N) % Ns;
+        ok = ok && sample >= 0;
+        stack = ok ? (sample / per_stack) % 2 : 0;
+        spec = ok ? (sample / N) % Ns : 0;
       }
-      long stride = (stack * Ns + spec) * (long)NN * N;
-      stride += (long)station * 38 * N;
-      for (uint32_t k = 0; k < snap; k++) stride += (long)kInputsPerSnap[k] * N;
-      stride += sample % N;
-      for (int j = 0; j < kInputsPerSnap[snap]; j++) {
-        long sb = 4 + 2L * ((long)i * kInputsPerSnap[snap] + j);
+      if (!ok) {
+        nsamples[2]++;
+        continue;
+      }
+      long stride = (stack * Ns + spec) * (long)NN * N + elem0 * N + sample % N;
+      for (int j = 0; j < nin; j++) {
+        long sb = 4 + 2L * ((long)i * nin + j);
         long dst = stride + (long)j * N;
         stacks[2 * dst] = orc_unpack_u32(buf[sb]);
         stacks[2 * dst + 1] = orc_unpack_u32(buf[sb + 1]);

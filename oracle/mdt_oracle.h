@@ -61,7 +61,9 @@ float orc_unpack_u32(uint32_t w);
 void orc_unpack_u32_array(const uint32_t *w, long n, float *out);
 int orc_count_unpack_mismatch_all(const float *table_lo16, long *first_bad);
 
-/* depacketiser, client.cu:846-911 (first block) and 945-1005 (steady state) */
+/* depacketiser, client.cu:846-911 (first block) and 945-1005 (steady state);
+ * nsamples[3]: samples to stack 0, to stack 1, and skipped (they would land
+ * outside the stacks) */
 void orc_depacketise(const uint32_t *pkts, int npkts, int words_per_pkt, int SperElement,
                      int Nelements_total, int N, int Ns, int T, int first_block,
                      float *stacks, long *nsamples);

@@ -211,7 +211,7 @@ def depacketise(pkts, SperElement, NN, N, Ns, T, first_block):
     pkts = _c(pkts, np.uint32)
     npk, words = pkts.shape
     stacks = np.zeros(2 * Ns * NN * N, np.complex64)
-    ns = np.zeros(2, np.int64)
+    ns = np.zeros(3, np.int64)  # samples to stack 0, to stack 1, skipped
     lib().orc_depacketise(_p(pkts), npk, words, SperElement, NN, N, Ns, T,
                           1 if first_block else 0, _p(stacks), _p(ns))
     return stacks.reshape(2, Ns, NN, N), ns

@@ -257,7 +257,7 @@ def test_depacketise_positions():
             assert st[0, 0, 30 + j, i] == complex(k, k + 1)
     for i in range(S):
         assert st[0, 1, 0, i] == complex(2 * i * 10, 2 * i * 10 + 1)
-    assert list(ns) == [2 * S, S]
+    assert list(ns) == [2 * S, S, 0]
 
 
 # ----------------------------------------------------------------- whole chain
