@@ -173,6 +173,27 @@ int mdt_depacketise(const uint32_t *pkts, int npkts, int words_per_pkt, int Sper
                     int NpolsxNelements, int Nchannels, int Nspectra, int Ntaps, int first_block,
                     mdt_complex *Stacks, unsigned long long *counts, hipStream_t stream);
 
+/* ---------------------------------------------------------------- egress (host)
+ * The client's CSV writer (MDT_OneStation_client.cu:1202-1272) and a binary
+ * dump, from visibilities in host memory.  Layouts: the reference's
+ * column-major cuBLAS output [Nchannels][NN][NN], or the native packed lower
+ * triangle [Nchannels][NN(NN+1)/2] (NN = Nelements * Npols).  CSV lines
+ * "j,p,k,q,z,re,im" for element j <= k, pol p <= q, channel z in fftshift
+ * order, V = alpha sum x_(j,p) conj(x_(k,q)), "%.6f".  all_pol_pairs = 0
+ * reproduces the reference's enumeration, which skips (j, 1, k, 0) for
+ * k > j; 1 writes those rows too.  mdt_csv_filename formats
+ * "MDT_%Y%m%d_%H%M%S.csv" in local time (client.cu:1209).  The binary dump
+ * is a 64-byte header (magic "MDTVIS01", int32 layout, dtype (0 complex64,
+ * 1 complex128), Nelements, Npols, Nchannels, Nspectra, int64 unix time) and
+ * the visibilities as stored. */
+#define MDT_VIS_FULL_COLMAJOR 0
+#define MDT_VIS_TRIANGLE 1
+int mdt_csv_filename(char *name, size_t size, long long unix_time);
+int mdt_write_csv(const char *path, const mdt_complex *vis, int layout, int Nelements, int Npols,
+                  int Nchannels, int all_pol_pairs);
+int mdt_write_binary(const char *path, const void *vis, int layout, int dtype, int Nelements, int Npols,
+                     int Nchannels, int Nspectra, long long unix_time);
+
 /* ---------------------------------------------------------------- native fused path
  * One dump: int8 (I,Q) samples [Ns+T-1][NN][N][2] -> fused PFB FIR + FFT +
  * corner turn -> channel-major X input [N][NN][Ns] complex FP32 (row and

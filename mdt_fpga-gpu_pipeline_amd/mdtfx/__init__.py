@@ -43,6 +43,8 @@ OP_N, OP_T, OP_C = 0, 1, 2
 XPREC_FP32 = 0
 XPREC_FP16_SPLIT = 1
 XPREC_FP8 = 2
+VIS_FULL_COLMAJOR = 0
+VIS_TRIANGLE = 1
 
 _lib = None
 
@@ -85,6 +87,9 @@ _SIGS = {
     "mdt_history_shift": [_VP, _I, _I, _I, _I, _I, _VP],
     "mdt_unpack_u32": [_VP, _VP, _SZ, _VP],
     "mdt_depacketise": [_VP, _I, _I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP],
+    "mdt_csv_filename": [ctypes.c_char_p, _SZ, ctypes.c_longlong],
+    "mdt_write_csv": [ctypes.c_char_p, _VP, _I, _I, _I, _I, _I],
+    "mdt_write_binary": [ctypes.c_char_p, _VP, _I, _I, _I, _I, _I, _I, ctypes.c_longlong],
     "mdt_fx_plan_create": [ctypes.POINTER(_VP), _I, _I, _I, _I, _I, _I],
     "mdt_fx_set_stream": [_VP, _VP],
     "mdt_fx_set_precision": [_VP, _I],
@@ -411,6 +416,65 @@ def depacketise(pkts, SperElement, NN, N, Ns, T, first_block, stacks=None, count
                                 1 if first_block else 0, _ptr(stacks, torch.complex64),
                                 _ptr(counts, torch.int64), _stream()))
     return stacks, counts
+
+
+def csv_filename(unix_time):
+    """MDT_%Y%m%d_%H%M%S.csv in local time (the client's output name)"""
+    buf = ctypes.create_string_buffer(64)
+    check(lib().mdt_csv_filename(buf, 64, int(unix_time)))
+    return buf.value.decode()
+
+
+def _host_array(a, dtype):
+    import numpy as np
+    a = np.ascontiguousarray(a)
+    if a.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {a.dtype}")
+    return a
+
+
+def write_csv(path, vis, layout, Nelements, Npols, Nchannels, all_pol_pairs=False):
+    """CSV in the client's format from host visibilities (numpy complex64):
+    layout VIS_FULL_COLMAJOR ([N][NN][NN] column-major, the reference's X
+    output) or VIS_TRIANGLE (the native packed triangle)."""
+    import numpy as np
+    v = _host_array(vis, np.complex64)
+    NN = Nelements * Npols
+    need = Nchannels * (NN * NN if layout == VIS_FULL_COLMAJOR else NN * (NN + 1) // 2)
+    if v.size < need:
+        raise ValueError("visibility array smaller than the layout needs")
+    check(lib().mdt_write_csv(str(path).encode(), ctypes.c_void_p(v.ctypes.data), layout, Nelements, Npols,
+                              Nchannels, 1 if all_pol_pairs else 0))
+
+
+def write_binary(path, vis, layout, Nelements, Npols, Nchannels, Nspectra, unix_time=0):
+    import numpy as np
+    v = np.ascontiguousarray(vis)
+    dtype = {np.dtype(np.complex64): 0, np.dtype(np.complex128): 1}.get(v.dtype)
+    if dtype is None:
+        raise TypeError("visibilities must be complex64 or complex128")
+    NN = Nelements * Npols
+    need = Nchannels * (NN * NN if layout == VIS_FULL_COLMAJOR else NN * (NN + 1) // 2)
+    if v.size < need:
+        raise ValueError("visibility array smaller than the layout needs")
+    check(lib().mdt_write_binary(str(path).encode(), ctypes.c_void_p(v.ctypes.data), layout, dtype, Nelements,
+                                 Npols, Nchannels, Nspectra, int(unix_time)))
+
+
+def read_binary(path):
+    """(header dict, visibilities) of a mdt_write_binary file"""
+    import numpy as np
+    raw = open(path, "rb").read()
+    if raw[:8] != b"MDTVIS01":
+        raise ValueError("not an MDT visibility dump")
+    layout, dtype, Ne, Np, Nc, Ns = np.frombuffer(raw, np.int32, 6, 8)
+    t = int(np.frombuffer(raw, np.int64, 1, 32)[0])
+    NN = int(Ne) * int(Np)
+    per = NN * NN if layout == VIS_FULL_COLMAJOR else NN * (NN + 1) // 2
+    data = np.frombuffer(raw, np.complex64 if dtype == 0 else np.complex128, int(Nc) * per, 64)
+    hdr = dict(layout=int(layout), dtype=int(dtype), Nelements=int(Ne), Npols=int(Np), Nchannels=int(Nc),
+               Nspectra=int(Ns), unix_time=t)
+    return hdr, data.reshape(int(Nc), per)
 
 
 def synth_i8(NN, N, nframes, seed=1, sigma=16.0, rho=0.0, max_delay=0, tones=(), t0=0, out=None):
