@@ -117,6 +117,43 @@ def fp8_leg(np, torch, mdtfx, steps, warmup, stream, err_channels, with_err):
     return out
 
 
+def stream_leg(torch, mdtfx, x8, dumps):
+    """The same C3 dumps through the streaming correlator: int8 stacks from
+    the two pinned host slots over PCIe (the history carried on the device),
+    F+X, visibilities back to pinned host memory, copies and compute on
+    separate streams.  Reports the host-inclusive rate beside the
+    device-resident value.  The host stacks are filled once; every dump
+    copies a full stack."""
+    plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
+    st = mdtfx.FxStream(plan)
+    for slot in range(2):
+        torch.from_numpy(st.stack(slot)).copy_(x8[T - 1:])
+    st.prime(x8[:T - 1].cpu().numpy())
+    acc = torch.zeros((N, plan.nb), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+
+    def run(n):
+        times = []
+        for d in range(n):
+            if d >= 2:
+                times.append(st.wait(d % 2))
+            st.submit(d % 2, acc)
+        for d in range(max(0, n - 2), n):
+            times.append(st.wait(d % 2))
+        return times
+
+    run(2)
+    t0 = time.perf_counter()
+    times = run(dumps)
+    wall = (time.perf_counter() - t0) * 1e3 / dumps
+    st.destroy()
+    mean = {k: sum(t[k] for t in times) / len(times) for k in times[0]}
+    stack_bytes = NS * NN * N * 2
+    return {"value": NS * N / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_dump": wall,
+            "dumps": dumps, "phase_ms": mean, "h2d_GB/s": stack_bytes / (mean["h2d_ms"] * 1e-3) / 1e9,
+            "stack_bytes": stack_bytes}
+
+
 def load_traffic():
     """HBM bytes per launch from the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
     runs, scaled by the copy-kernel calibration (profiles/traffic_c3.json)."""
@@ -183,6 +220,7 @@ def main():
     ap.add_argument("--err-channels", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 error legs")
     ap.add_argument("--no-fp8", action="store_true", help="skip the C4 FP8 leg")
+    ap.add_argument("--no-stream", action="store_true", help="skip the host-inclusive streaming leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -273,6 +311,9 @@ def main():
         "kernels": kernels,
         "roofline": roof,
     }
+    if not args.no_stream:
+        log("bench: streaming leg (host stacks over PCIe)")
+        out["stream_host_inclusive"] = stream_leg(torch, mdtfx, x8, max(4, min(args.steps, 8)))
     if not args.no_cpu:
         log("bench: CPU baseline + FP64 error legs")
         cpu, err = cpu_baseline_and_error(np, torch, plan, x8, vis16.cpu().numpy(), vis32.cpu().numpy(),

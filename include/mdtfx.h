@@ -241,7 +241,54 @@ int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNat
 int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents);
 /* the plan's power-of-two scale s of the FP16 sample formats */
 int mdt_fx_get_split_scale(mdt_fx_plan plan, float *scale);
+/* the plan's shape and precision (NULL outputs are skipped) */
+int mdt_fx_plan_shape(mdt_fx_plan plan, int *NpolsxNelements, int *Nchannels, int *Ntaps, int *Nspectra,
+                      int *xprec);
 int mdt_fx_plan_destroy(mdt_fx_plan plan);
+
+/* ---------------------------------------------------------------- streaming correlator
+ * Consecutive dumps through one plan, replacing the client's four-rank MPI
+ * loop (MDT_OneStation_client.cu:916-1276): the rank-1 ping-pong pinned
+ * stacks (client.cu:526, 1009-1057), the rank-2 F- and X-engines
+ * (client.cu:1083-1128), the rank-3 history shift (client.cu:1170) and the
+ * visibility read-back (client.cu:1199).  One process, three HIP streams:
+ * host-to-device copies, F+X compute and device-to-host copies overlap, ordered
+ * by events.
+ *
+ * Two host slots, each a pinned input stack of Nspectra new spectra (int8
+ * (I, Q) [Nspectra][NN][Nchannels][2], the F-engine's sample layout) and a
+ * pinned visibility buffer [Nchannels][NN(NN+1)/2] complex64 (the plan's
+ * triangle).  Each slot has its own device input buffer of Ntaps-1 + Nspectra
+ * spectra: a dump first copies the previous dump's last Ntaps-1 spectra to the
+ * front (the PFB history carry), then the slot's new stack behind them.  Before
+ * the first dump the history is what mdt_stream_prime loaded (zeros at
+ * creation).  Use: fill a slot's stack, mdt_stream_submit (returns at once),
+ * later mdt_stream_wait on that slot, read its visibilities, refill.
+ * Submitting a slot that has not been waited for since its last submit is an
+ * error.  The stream takes over the plan's stream; while the stream exists,
+ * use the plan only through it.  Needs Nspectra >= Ntaps-1. */
+typedef struct mdt_stream_s *mdt_stream;
+typedef struct {
+  float h2d_ms;     /* history carry + stack copy */
+  float fengine_ms; /* F-engine */
+  float xengine_ms; /* X-engine (+ FP64 accumulation) */
+  float d2h_ms;     /* visibility copy */
+  float total_ms;   /* start of the input copies to the end of the visibility copy */
+} mdt_dump_times;
+int mdt_stream_create(mdt_stream *stream, mdt_fx_plan plan);
+int mdt_stream_host_stack(mdt_stream stream, int slot, int8_t **stack);
+int mdt_stream_host_vis(mdt_stream stream, int slot, mdt_complex **vis);
+/* history = Ntaps-1 spectra (host memory, same layout as a stack), NULL for
+ * zeros; the next dump starts a new stream of spectra.  No dump may be in
+ * flight (the client's first block, client.cu:859-916). */
+int mdt_stream_prime(mdt_stream stream, const int8_t *history);
+/* Acc64: optional device FP64 accumulator [Nchannels][NN(NN+1)/2] complex,
+ * V added in (as mdt_fx_xengine) */
+int mdt_stream_submit(mdt_stream stream, int slot, double *Acc64);
+/* blocks until the slot's dump is done: its stack may be refilled and its
+ * visibilities read; times (optional) gets the dump's phase times */
+int mdt_stream_wait(mdt_stream stream, int slot, mdt_dump_times *times);
+int mdt_stream_destroy(mdt_stream stream);
 
 /* ---------------------------------------------------------------- calibration (a9)
  * No counterpart in the reference.  For raw triangle visibilities V (as from

@@ -263,6 +263,17 @@ int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents) {
                                plan->stream);
 }
 
+int mdt_fx_plan_shape(mdt_fx_plan plan, int *NpolsxNelements, int *Nchannels, int *Ntaps, int *Nspectra,
+                      int *xprec) {
+  MDT_REQUIRE_PTR(plan);
+  if (NpolsxNelements) *NpolsxNelements = plan->NN;
+  if (Nchannels) *Nchannels = plan->N;
+  if (Ntaps) *Ntaps = plan->T;
+  if (Nspectra) *Nspectra = plan->Ns;
+  if (xprec) *xprec = plan->xprec;
+  return MDT_SUCCESS;
+}
+
 int mdt_fx_get_split_scale(mdt_fx_plan plan, float *scale) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(scale);

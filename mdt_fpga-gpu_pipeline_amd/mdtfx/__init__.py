@@ -59,6 +59,11 @@ class _Complex(ctypes.Structure):
     _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float)]
 
 
+class DumpTimes(ctypes.Structure):
+    _fields_ = [("h2d_ms", ctypes.c_float), ("fengine_ms", ctypes.c_float), ("xengine_ms", ctypes.c_float),
+                ("d2h_ms", ctypes.c_float), ("total_ms", ctypes.c_float)]
+
+
 class SynthParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("sigma", ctypes.c_float), ("rho", ctypes.c_float),
                 ("max_delay", ctypes.c_int), ("ntones", ctypes.c_int),
@@ -103,6 +108,14 @@ _SIGS = {
     "mdt_fx_plan_destroy": [_VP],
     "mdt_fx_get_channel_exponents": [_VP, _VP],
     "mdt_fx_get_split_scale": [_VP, ctypes.POINTER(_F)],
+    "mdt_fx_plan_shape": [_VP, _VP, _VP, _VP, _VP, _VP],
+    "mdt_stream_create": [ctypes.POINTER(_VP), _VP],
+    "mdt_stream_host_stack": [_VP, _I, ctypes.POINTER(_VP)],
+    "mdt_stream_host_vis": [_VP, _I, ctypes.POINTER(_VP)],
+    "mdt_stream_prime": [_VP, _VP],
+    "mdt_stream_submit": [_VP, _I, _VP],
+    "mdt_stream_wait": [_VP, _I, _VP],
+    "mdt_stream_destroy": [_VP],
     "mdt_synth_i8": [ctypes.POINTER(SynthParams), _I, _I, _I, _VP, _VP],
     "mdt_fx_set_gains": [_VP, _VP, _VP],
     "mdt_calib_plan_create": [ctypes.POINTER(_VP), _I, _I],
@@ -390,6 +403,69 @@ class FxPlan:
     def destroy(self):
         if self._h:
             check(lib().mdt_fx_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class FxStream:
+    """Streaming correlator over consecutive dumps of one FxPlan: two host
+    slots, each a pinned int8 input stack [Ns][NN][N][2] and a pinned
+    visibility buffer [N][NN(NN+1)/2] complex64 (numpy views).  Fill a slot's
+    stack, submit(slot), later wait(slot) and read vis(slot).  The PFB history
+    (the previous dump's last T-1 spectra) is carried on the device."""
+
+    def __init__(self, plan):
+        import numpy as np
+        self._h = ctypes.c_void_p()
+        check(lib().mdt_stream_create(ctypes.byref(self._h), plan._h))
+        self.plan = plan  # the stream uses the plan until destroyed
+        self._stacks, self._vis = [], []
+        for slot in range(2):
+            p = ctypes.c_void_p()
+            check(lib().mdt_stream_host_stack(self._h, slot, ctypes.byref(p)))
+            buf = (ctypes.c_int8 * (plan.Ns * plan.NN * plan.N * 2)).from_address(p.value)
+            self._stacks.append(np.frombuffer(buf, np.int8).reshape(plan.Ns, plan.NN, plan.N, 2))
+            check(lib().mdt_stream_host_vis(self._h, slot, ctypes.byref(p)))
+            buf = (ctypes.c_float * (2 * plan.N * plan.nb)).from_address(p.value)
+            self._vis.append(np.frombuffer(buf, np.complex64).reshape(plan.N, plan.nb))
+
+    def stack(self, slot):
+        return self._stacks[slot]
+
+    def vis(self, slot):
+        return self._vis[slot]
+
+    def prime(self, history=None):
+        """history: int8 [T-1][NN][N][2] (host), None for zeros"""
+        import numpy as np
+        if history is None:
+            check(lib().mdt_stream_prime(self._h, None))
+            return
+        h = np.ascontiguousarray(history, dtype=np.int8)
+        if h.size < (self.plan.T - 1) * self.plan.NN * self.plan.N * 2:
+            raise ValueError("history needs T-1 spectra")
+        check(lib().mdt_stream_prime(self._h, ctypes.c_void_p(h.ctypes.data)))
+
+    def submit(self, slot, acc64=None):
+        if acc64 is not None and acc64.numel() < self.plan.N * self.plan.nb:
+            raise ValueError("accumulator smaller than N x NN(NN+1)/2")
+        check(lib().mdt_stream_submit(self._h, slot, _ptr(acc64, torch.complex128) if acc64 is not None else None))
+
+    def wait(self, slot):
+        """blocks until the slot's dump is done; returns its phase times (ms)"""
+        t = DumpTimes()
+        check(lib().mdt_stream_wait(self._h, slot, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in DumpTimes._fields_}
+
+    def destroy(self):
+        if self._h:
+            self._stacks, self._vis = [], []
+            check(lib().mdt_stream_destroy(self._h))
             self._h = ctypes.c_void_p()
 
     def __del__(self):
