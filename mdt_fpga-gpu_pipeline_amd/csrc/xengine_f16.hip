@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
 // of a wave mostly share their I rows; the I operand is reloaded only when
 // the row changes.
 constexpr int WC_NB = 8;                  // 32-row blocks (NN <= 256)
-constexpr int WC_WAVES = 8, WC_THREADS = WC_WAVES * 64, WC_BPW = 5;
+constexpr int WC_WAVES = 8, WC_THREADS = WC_WAVES * 64;
 constexpr int WC_KC = 16;                 // samples per K chunk
 constexpr int WC_ROWB = WC_KC * 4 + 16;   // bytes per row per plane (64 data + 16 pad)
 constexpr int WC_PLANE = WC_NB * 32 * WC_ROWB;
@@ -232,13 +232,16 @@ constexpr int WC_SEGS = WC_KC / 2;        // 16-byte segments (2 samples) per ro
 constexpr int WC_LOADS = WC_NB * 32 * WC_SEGS / WC_THREADS;  // per thread per chunk
 static_assert(WC_NB * 32 * WC_SEGS % WC_THREADS == 0, "chunk loads must divide among the threads");
 
-// block list (ra, rb), ra >= rb, 5 per wave for waves 0-3 and 4 (+ an unused
-// slot, ra = 255) for waves 4-7
-__constant__ unsigned char kWcBlocks[WC_WAVES * WC_BPW][2] = {
-    {7, 0}, {7, 1}, {7, 2}, {7, 3}, {7, 4},   {7, 5}, {7, 6}, {7, 7}, {6, 0}, {6, 1},
-    {6, 2}, {6, 3}, {6, 4}, {6, 5}, {6, 6},   {5, 0}, {5, 1}, {5, 2}, {5, 3}, {5, 4},
-    {5, 5}, {4, 0}, {4, 1}, {4, 2}, {255, 0}, {4, 3}, {4, 4}, {3, 0}, {3, 1}, {255, 0},
-    {3, 2}, {3, 3}, {2, 0}, {2, 1}, {255, 0}, {2, 2}, {1, 0}, {1, 1}, {0, 0}, {255, 0}};
+// Block assignment: rows pair up (7, 0), (6, 1), (5, 2), (4, 3), 9 blocks a
+// pair, one pair per SIMD.  Waves 0-3 ("A", one per SIMD) take the five
+// blocks (r, 0..4) of row r = 7 - wave: one I row, J rows known at compile
+// time, straight-line code with no per-block branch.  Waves 4-7 ("B") take
+// the remaining four blocks of their pair from kWcB, reloading I per block.
+constexpr int WC_A = 5, WC_B = 4;
+__constant__ unsigned char kWcB[4][WC_B][2] = {{{7, 5}, {7, 6}, {7, 7}, {0, 0}},
+                                               {{6, 5}, {6, 6}, {1, 0}, {1, 1}},
+                                               {{5, 5}, {2, 0}, {2, 1}, {2, 2}},
+                                               {{3, 0}, {3, 1}, {3, 2}, {3, 3}}};
 
 struct WcPrefetch {
   float4 v[WC_LOADS];
@@ -277,94 +280,65 @@ __device__ __forceinline__ void wc_stage(const WcPrefetch &p, char *lds) {
   }
 }
 
-// the MFMAs of one chunk (2 K-steps) over the wave's blocks
-__device__ __forceinline__ void wc_chunk(const char *bufp, const int (&ra)[WC_BPW], const int (&rb)[WC_BPW],
-                                         const bool (&act)[WC_BPW], int l32, int kh, f32x16 (&accR)[WC_BPW],
-                                         f32x16 (&accI)[WC_BPW]) {
+// the six products of one block and one K-step: real and imaginary parts of
+// hi*hi + hi*lo + lo*hi
+__device__ __forceinline__ void wc_block(half8 Ih, half8 Il, half8 Ph, half8 Pl, half8 Jh, half8 Jl, f32x16 &aR,
+                                         f32x16 &aI) {
+  aR = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jh, aR, 0, 0, 0);
+  aI = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jh, aI, 0, 0, 0);
+  aR = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jl, aR, 0, 0, 0);
+  aI = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jl, aI, 0, 0, 0);
+  aR = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il, Jh, aR, 0, 0, 0);
+  aI = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl, Jh, aI, 0, 0, 0);
+}
+
+// one chunk (2 K-steps) of an A wave: I row `irow`, J rows 32j + l32, j = 0..4
+__device__ __forceinline__ void wc_chunk_a(const char *bufp, int irow, int l32, int kh, f32x16 (&accR)[WC_A],
+                                           f32x16 (&accI)[WC_A]) {
 #pragma unroll
   for (int ks = 0; ks < WC_KC / 8; ks++) {
     const int koff = ks * 32 + kh * 16;
-    half8 Ih, Il, Ph, Pl;
-    int cached = -1;
+    const half8 Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
+    const half8 Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
+    const half8 Ph = rot_conj(Ih), Pl = rot_conj(Il);
 #pragma unroll
-    for (int b = 0; b < WC_BPW; b++) {
-      if (!act[b]) continue;
-      if (ra[b] != cached) {  // uniform: consecutive blocks mostly share the I row
-        const int irow = ra[b] * 32 + l32;
-        Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
-        Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
-        Ph = rot_conj(Ih);
-        Pl = rot_conj(Il);
-        cached = ra[b];
-      }
-      const int jrow = rb[b] * 32 + l32;
+    for (int j = 0; j < WC_A; j++) {
+      const int jrow = j * 32 + l32;
       const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
       const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
-      accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jh, accR[b], 0, 0, 0);
-      accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jh, accI[b], 0, 0, 0);
-      accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ih, Jl, accR[b], 0, 0, 0);
-      accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ph, Jl, accI[b], 0, 0, 0);
-      accR[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Il, Jh, accR[b], 0, 0, 0);
-      accI[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl, Jh, accI[b], 0, 0, 0);
+      wc_block(Ih, Il, Ph, Pl, Jh, Jl, accR[j], accI[j]);
     }
-    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-__global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
-  __shared__ __attribute__((aligned(16))) char lds[2 * WC_BUF];
-
-  const long nwg = gridDim.x;
-  const long w = blockIdx.x;
-  const int batch = (int)((nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w);
-  const float2 *base = g.Abase + (long)batch * g.Abatch;
-  const int NN = g.m, K = g.k;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int l32 = lane & 31, kh = lane >> 5;
-
-  int ra[WC_BPW], rb[WC_BPW];
-  bool act[WC_BPW];
+// one chunk of a B wave: blocks (ra[b], rb[b]), I reloaded per block
+__device__ __forceinline__ void wc_chunk_b(const char *bufp, const int (&ra)[WC_B], const int (&rb)[WC_B], int l32,
+                                           int kh, f32x16 (&accR)[WC_B], f32x16 (&accI)[WC_B]) {
 #pragma unroll
-  for (int b = 0; b < WC_BPW; b++) {
-    ra[b] = kWcBlocks[wave * WC_BPW + b][0];
-    rb[b] = kWcBlocks[wave * WC_BPW + b][1];
-    act[b] = ra[b] * 32 < NN;  // rb <= ra; the unused slot has ra = 255
-  }
-  f32x16 accR[WC_BPW], accI[WC_BPW];
+  for (int ks = 0; ks < WC_KC / 8; ks++) {
+    const int koff = ks * 32 + kh * 16;
 #pragma unroll
-  for (int b = 0; b < WC_BPW; b++)
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      accR[b][r] = 0.f;
-      accI[b][r] = 0.f;
-    }
-
-  // two-deep register prefetch: pa holds chunk c+1, pb chunk c+2 while chunk
-  // c is multiplied out of LDS
-  const int nchunk = (K + WC_KC - 1) / WC_KC;
-  WcPrefetch pa, pb;
-  wc_fetch(pa, base, g.ldI, NN, 0, K);
-  wc_stage(pa, lds);
-  if (nchunk > 1) wc_fetch(pa, base, g.ldI, NN, WC_KC, K);
-  if (nchunk > 2) wc_fetch(pb, base, g.ldI, NN, 2 * WC_KC, K);
-  for (int c = 0; c < nchunk; c += 2) {
-    __syncthreads();  // chunk c staged in buffer 0; buffer 1 is free
-    wc_chunk(lds, ra, rb, act, l32, kh, accR, accI);
-    if (c + 1 < nchunk) wc_stage(pa, lds + WC_BUF);
-    if (c + 3 < nchunk) wc_fetch(pa, base, g.ldI, NN, (c + 3) * WC_KC, K);
-    if (c + 1 < nchunk) {
-      __syncthreads();  // chunk c+1 staged in buffer 1; buffer 0 is free
-      wc_chunk(lds + WC_BUF, ra, rb, act, l32, kh, accR, accI);
-      if (c + 2 < nchunk) wc_stage(pb, lds);
-      if (c + 4 < nchunk) wc_fetch(pb, base, g.ldI, NN, (c + 4) * WC_KC, K);
+    for (int b = 0; b < WC_B; b++) {
+      const int irow = ra[b] * 32 + l32, jrow = rb[b] * 32 + l32;
+      const half8 Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
+      const half8 Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
+      const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
+      const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
+      wc_block(Ih, Il, rot_conj(Ih), rot_conj(Il), Jh, Jl, accR[b], accI[b]);
     }
   }
+}
 
+// V for blocks (ra[b], rb[b]): D[i][j] = V[j][i], so accumulator register r of
+// lane l holds V[32 ra + 8(r/4) + 4 kh + r%4][32 rb + l32]
+template <int NB>
+__device__ __forceinline__ void wc_epilogue(const XArgs &g, int batch, float oscale, const int (&ra)[NB],
+                                            const int (&rb)[NB], int l32, int kh, const f32x16 (&accR)[NB],
+                                            const f32x16 (&accI)[NB]) {
+  const int NN = g.m;
   double2 *acc = (double2 *)g.acc64;
 #pragma unroll
-  for (int b = 0; b < WC_BPW; b++) {
-    if (!act[b]) continue;
+  for (int b = 0; b < NB; b++) {
     const int bcol = rb[b] * 32 + l32;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -382,6 +356,78 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
       }
     }
     __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// the chunk loop of one wave kind: two-deep register prefetch (pa holds chunk
+// c+1, pb chunk c+2 while chunk c is multiplied out of LDS), LDS double buffer
+template <typename Chunk>
+__device__ __forceinline__ void wc_loop(char *lds, const float2 *base, long ld, int NN, int K, Chunk chunk) {
+  const int nchunk = (K + WC_KC - 1) / WC_KC;
+  WcPrefetch pa, pb;
+  wc_fetch(pa, base, ld, NN, 0, K);
+  wc_stage(pa, lds);
+  if (nchunk > 1) wc_fetch(pa, base, ld, NN, WC_KC, K);
+  if (nchunk > 2) wc_fetch(pb, base, ld, NN, 2 * WC_KC, K);
+  for (int c = 0; c < nchunk; c += 2) {
+    __syncthreads();  // chunk c staged in buffer 0; buffer 1 is free
+    chunk(lds);
+    if (c + 1 < nchunk) wc_stage(pa, lds + WC_BUF);
+    if (c + 3 < nchunk) wc_fetch(pa, base, ld, NN, (c + 3) * WC_KC, K);
+    if (c + 1 < nchunk) {
+      __syncthreads();  // chunk c+1 staged in buffer 1; buffer 0 is free
+      chunk(lds + WC_BUF);
+      if (c + 2 < nchunk) wc_stage(pb, lds);
+      if (c + 4 < nchunk) wc_fetch(pb, base, ld, NN, (c + 4) * WC_KC, K);
+    }
+  }
+}
+
+__global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
+  __shared__ __attribute__((aligned(16))) char lds[2 * WC_BUF];
+
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const int batch = (int)((nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w);
+  const float2 *base = g.Abase + (long)batch * g.Abatch;
+  const int NN = g.m, K = g.k;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l32 = lane & 31, kh = lane >> 5;
+
+  // blocks past NN (NN < 256) are multiplied out of zero rows and not stored
+  if (wave < 4) {
+    const int r = 7 - wave;
+    int ra[WC_A], rb[WC_A];
+    f32x16 accR[WC_A], accI[WC_A];
+#pragma unroll
+    for (int b = 0; b < WC_A; b++) {
+      ra[b] = r;
+      rb[b] = b;
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        accR[b][q] = 0.f;
+        accI[b][q] = 0.f;
+      }
+    }
+    const int irow = r * 32 + l32;
+    wc_loop(lds, base, g.ldI, NN, K, [&](const char *bufp) { wc_chunk_a(bufp, irow, l32, kh, accR, accI); });
+    wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
+  } else {
+    int ra[WC_B], rb[WC_B];
+    f32x16 accR[WC_B], accI[WC_B];
+#pragma unroll
+    for (int b = 0; b < WC_B; b++) {
+      ra[b] = kWcB[wave - 4][b][0];
+      rb[b] = kWcB[wave - 4][b][1];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        accR[b][q] = 0.f;
+        accI[b][q] = 0.f;
+      }
+    }
+    wc_loop(lds, base, g.ldI, NN, K, [&](const char *bufp) { wc_chunk_b(bufp, ra, rb, l32, kh, accR, accI); });
+    wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
   }
 }
 
