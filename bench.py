@@ -1,6 +1,6 @@
 """FX correlator benchmark on one MI355X.
 
-Workload (operating point C3): 256 inputs, 8-tap PFB, 4096-point FFT, 2048
+Workload (the scaled array): 256 inputs, 8-tap PFB, 4096-point FFT, 2048
 spectra per dump, int8 samples sigma=20 with a common component (rho=0.3)
 delayed per input by 0-8 samples; FP16 hi+lo split X-engine with FP32
 accumulation and the FP64 cross-dump accumulator.  A step is one dump: the
@@ -56,12 +56,12 @@ def time_path(torch, mdtfx, plan, x8, xin, vis, acc, steps, warmup):
 
 
 def fp8_leg(np, torch, mdtfx, steps, warmup, stream, err_channels, with_err):
-    """Operating point C4 (512 inputs, 4-tap PFB, 1024-pt FFT, 8192 spectra
+    """The wide-band shape (512 inputs, 4-tap PFB, 1024-pt FFT, 8192 spectra
     per dump, noise plus a tone ~40 dB above it) on the FP8 e4m3 X-engine with
     per-channel power-of-two RMS scaling; full dump resident in HBM."""
     NN4, N4, T4, NS4 = 512, 1024, 4, 8192
     tone_bin, tone_amp, sigma = 300, 12.0, 4.0
-    log(f"bench: FP8 path, C4 NN={NN4} N={N4} T={T4} Ns={NS4}")
+    log(f"bench: FP8 path, wide-band NN={NN4} N={N4} T={T4} Ns={NS4}")
     plan = mdtfx.FxPlan(NN4, N4, T4, NS4, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP8)
     plan.set_stream(stream)
     x8 = mdtfx.synth_i8(NN4, N4, NS4 + T4 - 1, seed=SEED, sigma=sigma, rho=RHO, max_delay=MAXDELAY,
@@ -74,7 +74,7 @@ def fp8_leg(np, torch, mdtfx, steps, warmup, stream, err_channels, with_err):
     f_bytes = NN4 * (NS4 + T4 - 1) * N4 * 2 + T4 * N4 * 4 + NN4 * NS4 * N4 * 4
     x_flops = 8.0 * N4 * NS4 * nb4
     out = {
-        "workload": "C4 wide-band: 512 inputs, 4-tap PFB, 1024-pt FFT, 8192 spectra/dump, FP8 e4m3 "
+        "workload": "wide-band: 512 inputs, 4-tap PFB, 1024-pt FFT, 8192 spectra/dump, FP8 e4m3 "
                     "X-engine with per-channel power-of-two RMS scaling, FP32 accumulate",
         "data": f"synthetic int8 sigma={sigma}, rho={RHO}, tone at bin {tone_bin} amplitude {tone_amp}, seed {SEED}",
         "value": NS4 * N4 / (wall * 1e-3) / 1e6,
@@ -118,7 +118,7 @@ def fp8_leg(np, torch, mdtfx, steps, warmup, stream, err_channels, with_err):
 
 
 def stream_leg(torch, mdtfx, x8, dumps):
-    """The same C3 dumps through the streaming correlator: int8 stacks from
+    """The same scaled-array dumps through the streaming correlator: int8 stacks from
     the two pinned host slots over PCIe (the history carried on the device),
     F+X, visibilities back to pinned host memory, copies and compute on
     separate streams.  Reports the host-inclusive rate beside the
@@ -156,8 +156,8 @@ def stream_leg(torch, mdtfx, x8, dumps):
 
 def load_traffic():
     """HBM bytes per launch from the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-    runs, scaled by the copy-kernel calibration (profiles/traffic_c3.json)."""
-    p = os.path.join(ROOT, "profiles", "traffic_c3.json")
+    runs, scaled by the copy-kernel calibration (profiles/traffic_fx_dumps.json)."""
+    p = os.path.join(ROOT, "profiles", "traffic_fx_dumps.json")
     if not os.path.exists(p):
         return {}
     with open(p) as f:
@@ -219,7 +219,7 @@ def main():
     ap.add_argument("--cpu-spectra", type=int, default=2048)
     ap.add_argument("--err-channels", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 error legs")
-    ap.add_argument("--no-fp8", action="store_true", help="skip the C4 FP8 leg")
+    ap.add_argument("--no-fp8", action="store_true", help="skip the wide-band FP8 leg")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-inclusive streaming leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (HBM counter calibration)")
@@ -245,7 +245,7 @@ def main():
         print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
         return
     stream = torch.cuda.current_stream().cuda_stream
-    log(f"bench: C3 NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
+    log(f"bench: scaled array NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
     plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
     plan.set_stream(stream)
     x8 = mdtfx.synth_i8(NN, N, NS + T - 1, seed=SEED, sigma=SIGMA, rho=RHO, max_delay=MAXDELAY)
@@ -302,7 +302,7 @@ def main():
         "dtype": "fp16-split",
         "data": f"synthetic int8 (I,Q): sigma={SIGMA}, common component rho={RHO} delayed 0-{MAXDELAY} "
                 f"samples per input, seed {SEED}",
-        "config": {"workload": "C3 scaled array: 256 inputs, 8-tap PFB (BH4-windowed sinc), 4096-pt FFT, "
+        "config": {"workload": "scaled array: 256 inputs, 8-tap PFB (BH4-windowed sinc), 4096-pt FFT, "
                                "2048 spectra/dump, FP16 hi+lo split X-engine, FP32 accumulate, FP64 cross-dump "
                                "accumulator",
                    "inputs": NN, "taps": T, "channels": N, "spectra_per_dump": NS, "baselines": nb},
@@ -323,7 +323,7 @@ def main():
     else:
         out["cpu_baseline"] = None
     if not args.no_fp8:
-        out["fp8_c4"] = fp8_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), stream,
+        out["fp8_wideband"] = fp8_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), stream,
                                 4, not args.no_cpu)
     print(json.dumps(out), flush=True)
 

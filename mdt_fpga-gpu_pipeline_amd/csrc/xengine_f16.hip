@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
 }
 
 // ---------------------------------------------------------------------------
-// Whole-channel kernel for 128 < NN <= 256 (the C3 shape): one workgroup per
+// Whole-channel kernel for 128 < NN <= 256 (the 256-input scaled array): one workgroup per
 // channel computes every 32 x 32 block of the lower triangle (36 blocks for
 // NN = 256), so each sample is read from HBM exactly once (128 x 128 tiles
 // read a 256-input channel twice).  8 waves, two per SIMD with 256 registers

@@ -29,7 +29,7 @@ extern "C" {
 
 /* window kinds */
 #define ORC_WINDOW_BH4 0     /* sinc x minimum 4-term Blackman-Harris, devfn.cu:52-88 */
-#define ORC_WINDOW_HAMMING 1 /* sinc x periodic Hamming (operating-point C1 variant) */
+#define ORC_WINDOW_HAMMING 1 /* sinc x periodic Hamming (the 40-input Hamming case) */
 
 double orc_sinpi(double x);
 double orc_cospi(double x);

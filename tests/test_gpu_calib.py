@@ -1,4 +1,4 @@
-"""Gain/delay calibration (operating point C5: 64 inputs, 4-tap PFB, 1024
+"""Gain/delay calibration (the calibration case: 64 inputs, 4-tap PFB, 1024
 channels, 512 spectra): the F-engine gain stage, the GPU loss and the
 hand-written adjoint, against the FP64 oracle (analytic gradient and central
 finite differences)."""

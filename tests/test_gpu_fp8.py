@@ -1,4 +1,4 @@
-"""FP8 (OCP e4m3) X-engine with per-channel power-of-two RMS scaling: the C4
+"""FP8 (OCP e4m3) X-engine with per-channel power-of-two RMS scaling: the wide-band
 shape (512 inputs, 4-tap PFB, 1024 channels) at a reduced integration length,
 and a ragged shape (partial tiles, partial K chunk), with a tone 40 dB above
 the noise.
@@ -68,7 +68,7 @@ def _run(NN, N, T, Ns, tone_bin, tone_amp, sigma, seed):
 
 CASES = [
     # NN, N, T, Ns, tone bin, tone amplitude, sigma
-    (512, 1024, 4, 64, 300, 40.0, 4.0),   # C4 shape, 64 of its 8192 spectra
+    (512, 1024, 4, 64, 300, 40.0, 4.0),   # wide-band shape, 64 of its 8192 spectra
     (96, 512, 8, 100, 77, 40.0, 4.0),     # partial 128-row tile, 100 = 3 x 32 + 4 spectra
 ]
 

@@ -45,7 +45,7 @@ def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns):
 
 
 def test_native_chain_c1_vs_fp64():
-    """Operating point C1: 40 inputs, 4-tap Hamming PFB, 1024 channels, 256
+    """The 40-input Hamming case: 40 inputs, 4-tap Hamming PFB, 1024 channels, 256
     spectra, int8 N(0, 16^2) + 2 tones -> 820 baselines x 1024 channels."""
     NN, N, T, Ns = 40, 1024, 4, 256
     plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_HAMMING)
@@ -56,7 +56,7 @@ def test_native_chain_c1_vs_fp64():
     W = _host(plan.window()).astype(np.float64)
     v64 = orc.fx_chain_f64(W, _host(x8), NN, N, T, Ns, np.arange(N))
     err = _tri_autonorm_err(V, v64, NN)
-    print(f"C1 max autocorrelation-normalised error vs FP64: {err:.3e}")
+    print(f"40-input case: max autocorrelation-normalised error vs FP64: {err:.3e}")
     assert err < 1e-5
     # FP32 port on the CPU agrees to the same bar
     v32 = orc.fx_chain_f32(W.astype(np.float32), _host(x8), NN, N, T, Ns)
@@ -140,7 +140,7 @@ def test_synth_deterministic_and_statistics():
 
 
 def test_synth_beyond_32bit_grid():
-    """the C4 input (8195 frames x 512 inputs x 1024 channels = 4.3e9 samples,
+    """the wide-band input (8195 frames x 512 inputs x 1024 channels = 4.3e9 samples,
     more than a 32-bit dispatch grid holds): every frame is generated, and
     frame f equals a one-frame synthesis started at t0 = f N"""
     NN, N, F = 512, 1024, 8195
@@ -158,7 +158,7 @@ def test_synth_beyond_32bit_grid():
 @pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (256, 4096, 8, 64), (130, 512, 8, 100), (3, 64, 4, 9)])
 def test_fp16_split_vs_fp64(NN, N, T, Ns):
     """FP16 hi+lo split X-engine within the 1e-5 autocorrelation-normalised bar
-    (operating point C3 shape at reduced integration length)."""
+    (the 256-input scaled-array shape at reduced integration length)."""
     print(f"fp16split NN={NN} N={N} T={T} Ns={Ns}", flush=True)
     plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
     x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=NN + N, sigma=20.0, rho=0.3, max_delay=8,
