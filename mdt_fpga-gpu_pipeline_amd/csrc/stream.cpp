@@ -163,10 +163,8 @@ int mdt_stream_prime(mdt_stream stream, const int8_t *history) {
   return MDT_SUCCESS;
 }
 
-int mdt_stream_submit(mdt_stream stream, int slot, double *Acc64) {
-  MDT_TRY(check_slot(stream, slot));
-  mdt_stream s = stream;
-  if (s->busy[slot]) return fail(MDT_ERR_INVALID_VALUE, "slot %d submitted again before mdt_stream_wait", slot);
+// the enqueue sequence of one dump (see the top of this file)
+static int enqueue(mdt_stream s, int slot, double *Acc64) {
   Ev &e = s->ev[slot];
   const size_t hist = s->spectrum * (s->T - 1), stack = s->spectrum * s->Ns;
   const int8_t *src = s->prev < 0 ? s->prime : s->in[s->prev] + stack;  // previous dump's last T-1 spectra
@@ -188,6 +186,22 @@ int mdt_stream_submit(mdt_stream stream, int slot, double *Acc64) {
   MDT_HIP_TRY(hipStreamWaitEvent(s->cout, e.x_done, 0));
   MDT_HIP_TRY(hipMemcpyAsync(s->host_vis[slot], s->vis[slot], s->vis_bytes, hipMemcpyDeviceToHost, s->cout));
   MDT_HIP_TRY(hipEventRecord(e.vis_free, s->cout));
+  return MDT_SUCCESS;
+}
+
+int mdt_stream_submit(mdt_stream stream, int slot, double *Acc64) {
+  MDT_TRY(check_slot(stream, slot));
+  mdt_stream s = stream;
+  if (s->busy[slot]) return fail(MDT_ERR_INVALID_VALUE, "slot %d submitted again before mdt_stream_wait", slot);
+  const int st = enqueue(s, slot, Acc64);
+  if (st != MDT_SUCCESS) {
+    // part of the dump may be queued: let it drain, so that the slot's
+    // buffers are free again when this returns (the error text is kept)
+    (void)hipStreamSynchronize(s->cin);
+    (void)hipStreamSynchronize(s->comp);
+    (void)hipStreamSynchronize(s->cout);
+    return st;
+  }
   s->busy[slot] = true;
   s->prev = slot;
   return MDT_SUCCESS;
