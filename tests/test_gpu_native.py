@@ -155,7 +155,8 @@ def test_synth_beyond_32bit_grid():
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (256, 4096, 8, 64), (130, 512, 8, 100), (3, 64, 4, 9)])
+@pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (256, 4096, 8, 64), (130, 512, 8, 100), (3, 64, 4, 9),
+                                       (300, 256, 4, 40)])  # > 256 inputs: 128-input tiles, ragged
 def test_fp16_split_vs_fp64(NN, N, T, Ns):
     """FP16 hi+lo split X-engine within the 1e-5 autocorrelation-normalised bar
     (the 256-input scaled-array shape at reduced integration length)."""
