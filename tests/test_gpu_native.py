@@ -44,7 +44,7 @@ def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns):
     assert np.array_equal(got.view(np.uint32), np.ascontiguousarray(ref).view(np.uint32))
 
 
-def test_native_chain_c1_vs_fp64():
+def test_native_chain_40in_hamming_vs_fp64():
     """The 40-input Hamming case: 40 inputs, 4-tap Hamming PFB, 1024 channels, 256
     spectra, int8 N(0, 16^2) + 2 tones -> 820 baselines x 1024 channels."""
     NN, N, T, Ns = 40, 1024, 4, 256
