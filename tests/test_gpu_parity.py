@@ -52,6 +52,24 @@ def test_pps_batch_bit_exact(N, T, NN, Ns):
     assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
 
 
+def test_pps_batch_i8_all_pairs():
+    """every int8 (I, Q) pair: the int8 FIR (the F-engine's input stage) ==
+    the float FIR (PpS_Batch, MDT_device_functions.cu:113-161) on the same
+    values, bit for bit; 5 frames, each a different permutation of the 2^16
+    pairs over 4 inputs x 16384 channels"""
+    N, T, NN, Ns = 16384, 4, 4, 2
+    rng = _rng(16)
+    pairs = np.stack(np.meshgrid(np.arange(-128, 128), np.arange(-128, 128), indexing="ij"), -1).reshape(-1, 2)
+    x8 = np.stack([pairs[rng.permutation(65536)] for _ in range(Ns + T - 1)]).astype(np.int8).reshape(Ns + T - 1, NN, N, 2)
+    W = mdtfx.pfb_window(N, T, mdtfx.WINDOW_BH4)
+    y8 = _host(mdtfx.pps_batch_i8(W, _to_gpu(x8), NN, Ns))
+    xf = (x8[..., 0].astype(np.float32) + 1j * x8[..., 1].astype(np.float32)).astype(np.complex64)
+    yf = _host(mdtfx.pps_batch(W, _to_gpu(xf), NN, Ns))
+    assert np.array_equal(y8.view(np.uint32), yf.view(np.uint32))
+    ref = orc.pps_i8_f32(_host(W), x8, NN, Ns)
+    assert np.array_equal(y8.view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("N,T,NN,Ns", [(64, 4, 3, 5), (1024, 8, 5, 40), (4096, 8, 3, 9), (128, 6, 3, 4)])
 def test_pps_batch_i8_bit_exact(N, T, NN, Ns):
     print(f"pps_i8 N={N} T={T} NN={NN} Ns={Ns}", flush=True)

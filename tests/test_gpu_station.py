@@ -68,4 +68,14 @@ def test_station_shape_dropin_eight_dumps():
             e64 = _tri_autonorm_err(Vd[chans], v64)
             msg += f", drop-in vs FP64 {e64:.3e}"
             assert e64 < 1e-5, msg
+        if d == 0:
+            # x = sqrt(1 - rho) n + sqrt(rho) c(t - d_e): the normalised cross
+            # power of every baseline is ~rho; Cauchy-Schwarz holds everywhere
+            diag = np.array([orc.tri_index(i, i) for i in range(NN)])
+            auto = np.real(Vd[:, diag]).astype(np.float64)
+            coef = np.abs(Vd) / np.sqrt(auto[:, a] * auto[:, b])
+            assert np.all(coef <= 1.0 + 1e-6)
+            rho = float(coef[:, a != b].mean())
+            msg += f", mean normalised cross power {rho:.4f}"
+            assert abs(rho - 0.3) < 0.01, msg
         print(msg, flush=True)
