@@ -429,15 +429,23 @@ class FxStream:
             p = ctypes.c_void_p()
             check(lib().mdt_stream_host_stack(self._h, slot, ctypes.byref(p)))
             buf = (ctypes.c_int8 * (plan.Ns * plan.NN * plan.N * 2)).from_address(p.value)
+            buf._owner = self  # a view keeps the stream (and its pinned memory) alive
             self._stacks.append(np.frombuffer(buf, np.int8).reshape(plan.Ns, plan.NN, plan.N, 2))
             check(lib().mdt_stream_host_vis(self._h, slot, ctypes.byref(p)))
             buf = (ctypes.c_float * (2 * plan.N * plan.nb)).from_address(p.value)
+            buf._owner = self
             self._vis.append(np.frombuffer(buf, np.complex64).reshape(plan.N, plan.nb))
 
     def stack(self, slot):
+        """pinned int8 input stack of a slot (valid until destroy())"""
+        if not self._h:
+            raise MdtError(ERR_NULL_POINTER, "stream destroyed")
         return self._stacks[slot]
 
     def vis(self, slot):
+        """pinned visibilities of a slot (valid until destroy())"""
+        if not self._h:
+            raise MdtError(ERR_NULL_POINTER, "stream destroyed")
         return self._vis[slot]
 
     def prime(self, history=None):
@@ -463,6 +471,8 @@ class FxStream:
         return {k: getattr(t, k) for k, _ in DumpTimes._fields_}
 
     def destroy(self):
+        """frees the pinned slots: views from stack() and vis() must not be
+        used afterwards"""
         if self._h:
             self._stacks, self._vis = [], []
             check(lib().mdt_stream_destroy(self._h))

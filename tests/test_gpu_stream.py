@@ -96,5 +96,7 @@ def test_stream_slot_errors():
         st.submit(2)
     st.wait(0)
     st.destroy()
+    with pytest.raises(mdtfx.MdtError):  # the pinned slots are gone
+        st.stack(0)
     with pytest.raises(mdtfx.MdtError):  # Ns < T-1
         mdtfx.FxStream(mdtfx.FxPlan(16, 256, 8, 4))
