@@ -165,6 +165,15 @@ def _ptr(t, dtype=None):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _need(t, n, what, esize=None):
+    """the kernels take raw pointers: refuse a buffer smaller than the extent
+    they touch, or of the wrong element size, before anything is launched"""
+    if esize is not None and t.element_size() != esize:
+        raise TypeError(f"{what}: {esize}-byte elements expected, got {t.element_size()}")
+    if t.numel() < n:
+        raise ValueError(f"{what}: {t.numel()} elements, the call touches {n}")
+
+
 def version():
     v = ctypes.c_int()
     check(lib().mdt_version(ctypes.byref(v)))
@@ -185,6 +194,8 @@ def pps_batch(W, x, NN, Ns, out=None):
     T, N = W.shape
     if out is None:
         out = torch.empty((Ns, NN, N), dtype=torch.complex64, device="cuda")
+    _need(x, (Ns + T - 1) * NN * N, "input")
+    _need(out, Ns * NN * N, "output")
     check(lib().mdt_pps_batch(N, T, _ptr(W, torch.float32), _ptr(x, torch.complex64),
                               _ptr(out, torch.complex64), NN, Ns, _stream()))
     return out
@@ -195,6 +206,8 @@ def pps_batch_i8(W, x8, NN, Ns, out=None):
     T, N = W.shape
     if out is None:
         out = torch.empty((Ns, NN, N), dtype=torch.complex64, device="cuda")
+    _need(x8, (Ns + T - 1) * NN * N * 2, "input")
+    _need(out, Ns * NN * N, "output")
     check(lib().mdt_pps_batch_i8(N, T, _ptr(W, torch.float32), _ptr(x8, torch.int8),
                                  _ptr(out, torch.complex64), NN, Ns, _stream()))
     return out
@@ -213,6 +226,8 @@ class FftPlan:
         check(lib().mdt_fft_set_stream(self._h, ctypes.c_void_p(stream_handle)))
 
     def exec_c2c(self, idata, odata, direction=FFT_FORWARD):
+        _need(idata, self.nx * self.batch, "idata")
+        _need(odata, self.nx * self.batch, "odata")
         check(lib().mdt_fft_exec_c2c(self._h, _ptr(idata, torch.complex64),
                                      _ptr(odata, torch.complex64), direction))
         return odata
@@ -233,6 +248,8 @@ class FftPlan:
 def reorder_foutput(F, NN, N, Ns, out=None):
     if out is None:
         out = torch.empty((N, Ns, NN), dtype=F.dtype, device="cuda")
+    _need(F, Ns * NN * N, "FOutput", 8)
+    _need(out, Ns * NN * N, "XInput", 8)
     check(lib().mdt_reorder_foutput(NN, N, _ptr(F), _ptr(out), Ns, _stream()))
     return out
 
@@ -251,6 +268,8 @@ class XEngine:
     def cgemm_batched(self, transa, transb, m, n, k, alpha, Aptrs, lda, Bptrs, ldb, beta,
                       Cptrs, ldc, batch):
         """A/B/C pointer tables: int64 device tensors of data_ptr() values."""
+        for t, name in ((Aptrs, "A table"), (Bptrs, "B table"), (Cptrs, "C table")):
+            _need(t, batch, name)
         al = _Complex(float(complex(alpha).real), float(complex(alpha).imag))
         be = _Complex(float(complex(beta).real), float(complex(beta).imag))
         check(lib().mdt_cgemm_batched(self._h, transa, transb, m, n, k, ctypes.byref(al),
@@ -258,6 +277,8 @@ class XEngine:
                                       ctypes.byref(be), _ptr(Cptrs, torch.int64), ldc, batch))
 
     def herk(self, NN, Ns, alpha, XIn, XOut, N):
+        _need(XIn, N * Ns * NN, "XIn")
+        _need(XOut, N * NN * NN, "XOut")
         check(lib().mdt_xengine_herk(self._h, NN, Ns, float(alpha), _ptr(XIn, torch.complex64),
                                      _ptr(XOut, torch.complex64), N))
         return XOut
@@ -277,6 +298,7 @@ class XEngine:
 # ------------------------------------------------------------------ a6, a7
 def history_shift(FInput, NN, N, T, Ns):
     eb = 8 if FInput.dtype == torch.complex64 else 2
+    _need(FInput, (Ns + T - 1) * NN * N * (1 if eb == 8 else 2), "FInput", 8 if eb == 8 else 1)
     check(lib().mdt_history_shift(_ptr(FInput), NN, N, T, Ns, eb, _stream()))
 
 
@@ -284,6 +306,8 @@ def unpack_u32(words, out=None):
     """words: int32 tensor holding the raw uint32 bit patterns."""
     if out is None:
         out = torch.empty(words.shape, dtype=torch.float32, device="cuda")
+    _need(words, words.numel(), "words", 4)
+    _need(out, words.numel(), "output")
     check(lib().mdt_unpack_u32(_ptr(words), _ptr(out, torch.float32), words.numel(), _stream()))
     return out
 

@@ -106,3 +106,24 @@ def test_python_front_end_refuses_cpu_tensors():
     x = torch.zeros((7, 3, 64), dtype=torch.complex64)
     with pytest.raises(ValueError):
         mdtfx.pps_batch(W, x, 3, 4, out=torch.zeros((4, 3, 64), dtype=torch.complex64))
+
+
+def test_python_front_end_checks_buffer_sizes():
+    """raw-pointer calls refuse short or wrongly typed buffers before any
+    launch (the checks run on the host, so CPU tensors exercise them)"""
+    import torch
+    import mdtfx
+    W = torch.zeros((4, 64), dtype=torch.float32)
+    with pytest.raises(ValueError, match="input"):
+        mdtfx.pps_batch(W, torch.zeros((6, 3, 64), dtype=torch.complex64), 3, 4,
+                        out=torch.zeros((4, 3, 64), dtype=torch.complex64))  # needs Ns+T-1 = 7 spectra
+    with pytest.raises(ValueError, match="output"):
+        mdtfx.pps_batch_i8(W, torch.zeros((7, 3, 64, 2), dtype=torch.int8), 3, 4,
+                           out=torch.zeros((3, 3, 64), dtype=torch.complex64))
+    with pytest.raises(TypeError):
+        mdtfx.reorder_foutput(torch.zeros((4, 3, 64), dtype=torch.float32), 3, 64, 4,
+                              out=torch.zeros((64, 4, 3), dtype=torch.float32))
+    with pytest.raises(TypeError):
+        mdtfx.unpack_u32(torch.zeros(16, dtype=torch.int16), out=torch.zeros(16))
+    with pytest.raises(ValueError):
+        mdtfx.history_shift(torch.zeros((5, 3, 64), dtype=torch.complex64), 3, 64, 4, 4)  # needs 7 spectra
