@@ -164,6 +164,16 @@ def load_traffic():
         return json.load(f)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, err_channels):
     """The oracle's two legs on the host: (1) the FP32 CPU port of the chain
     timed on a bounded sample (all inputs and channels, the first
@@ -180,6 +190,11 @@ def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, er
     t0 = time.perf_counter()
     orc.fx_chain_f32(W, sub, NN, N, T, sample_spectra, nthreads)
     dt = time.perf_counter() - t0
+    # one core, on a sample 1/16 the size
+    s1 = max(2, sample_spectra // 16)
+    t0 = time.perf_counter()
+    orc.fx_chain_f32(W, np.ascontiguousarray(x_host[: s1 + T - 1]), NN, N, T, s1, 1)
+    dt1 = time.perf_counter() - t0
     cpu = {
         "value": sample_spectra * N / dt / 1e6,
         "unit": "Msamples/s/input",
@@ -188,6 +203,8 @@ def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, er
         "sample": f"FP32 CPU restatement of the chain (fmaf FIR, radix-2 FFT, corner turn, "
                   f"FP32 triangle correlation) on all {NN} inputs x {N} channels, first "
                   f"{sample_spectra} of {NS} spectra of the same dump; {dt:.1f} s",
+        "single_core": {"value": s1 * N / dt1 / 1e6, "sample_spectra": s1, "seconds": round(dt1, 1)},
+        "host": {"cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count()},
     }
     chans = np.unique(np.linspace(0, N - 1, err_channels).astype(int))
     t0 = time.perf_counter()
