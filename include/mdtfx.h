@@ -85,6 +85,11 @@ int mdt_version(int *version);
  * array of Ntaps*Nchannels floats, tap-major.  Values are evaluated in FP64
  * and rounded once to FP32. */
 int mdt_pfb_window(int Nchannels, int Ntaps, float *Window, int kind, hipStream_t stream);
+/* The name the client calls, PFB_Window_ExBlackman<<<...>>>(Nchannels, Ntaps,
+ * d_Window) (MDT_OneStation_client.cu:589), which the reference never defines;
+ * the window it provides is the minimum 4-term Blackman-Harris one, so this is
+ * mdt_pfb_window(..., MDT_WINDOW_BH4, stream). */
+int mdt_pfb_window_exblackman(int Nchannels, int Ntaps, float *Window, hipStream_t stream);
 
 /* ---------------------------------------------------------------- a2 PFB FIR
  * Replaces PpS_Batch<<<(ceil(N/1024), Nspectra, NN), 1024>>>(Nchannels, Ntaps,

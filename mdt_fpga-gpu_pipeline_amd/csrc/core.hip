@@ -242,6 +242,10 @@ int mdt_pfb_window(int Nchannels, int Ntaps, float *Window, int kind, hipStream_
   return launch_window(Nchannels, Ntaps, Window, kind, stream);
 }
 
+int mdt_pfb_window_exblackman(int Nchannels, int Ntaps, float *Window, hipStream_t stream) {
+  return mdt_pfb_window(Nchannels, Ntaps, Window, MDT_WINDOW_BH4, stream);
+}
+
 int mdt_reorder_foutput(int NpolsxNelements, int Nchannels, const mdt_complex *FOutput,
                         mdt_complex *XInput, int Nspectra, hipStream_t stream) {
   MDT_TRY(check_inputs(NpolsxNelements));

@@ -91,6 +91,7 @@ _SIGS = {
     "mdt_xengine_herk": [_VP, _I, _I, _F, _VP, _VP, _I],
     "mdt_history_shift": [_VP, _I, _I, _I, _I, _I, _VP],
     "mdt_unpack_u32": [_VP, _VP, _SZ, _VP],
+    "mdt_pfb_window_exblackman": [_I, _I, _VP, _VP],
     "mdt_depacketise": [_VP, _I, _I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP],
     "mdt_csv_filename": [ctypes.c_char_p, _SZ, ctypes.c_longlong],
     "mdt_write_csv": [ctypes.c_char_p, _VP, _I, _I, _I, _I, _I],
@@ -181,6 +182,15 @@ def version():
 
 
 # ------------------------------------------------------------------ a1
+def pfb_window_exblackman(N, T, out=None):
+    """the client's PFB_Window_ExBlackman call (the BH4 window)"""
+    if out is None:
+        out = torch.empty((T, N), dtype=torch.float32, device="cuda")
+    _need(out, T * N, "Window")
+    check(lib().mdt_pfb_window_exblackman(N, T, _ptr(out, torch.float32), _stream()))
+    return out
+
+
 def pfb_window(N, T, kind=WINDOW_BH4, out=None):
     if out is None:
         out = torch.empty((T, N), dtype=torch.float32, device="cuda")

@@ -39,6 +39,14 @@ def test_window_vs_oracle(N, T, kind):
     assert W.ravel()[TN // 2] == 1.0
 
 
+def test_window_exblackman_name():
+    """the client's PFB_Window_ExBlackman call (client.cu:589) is the BH4 window"""
+    for N, T in ((1024, 8), (2048, 4)):
+        a = _host(mdtfx.pfb_window_exblackman(N, T))
+        b = _host(mdtfx.pfb_window(N, T, mdtfx.WINDOW_BH4))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 # ----------------------------------------------------------------- a2 FIR
 @pytest.mark.parametrize("N,T,NN,Ns", [(64, 4, 3, 5), (1024, 8, 5, 40), (96, 5, 4, 33), (256, 16, 3, 70),
                                        (2048, 4, 6, 64)])
