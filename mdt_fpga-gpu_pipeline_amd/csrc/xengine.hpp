@@ -33,6 +33,51 @@ struct XArgs {
   float in_scale; // FP16 split: exact power-of-two pre-scale of the FP32 samples
 };
 
+// Triangle epilogue of one 32x32 MFMA block: register r of the lane holds
+// the block element (a = a0 + 8(r/4) + 4 kh + r%4, b) whose value is
+// alpha' * (dre[r], -dim[r]) (D[i][j] = V[j][i] = conj of V[i][j]).  Stores
+// V[a][b] at cb + a(a+1)/2 + b where a >= b and a, b < NN, and adds it into
+// the FP64 accumulator (nullable).  The accumulator is read in groups of G
+// loads issued before the first is used -- 16/G memory latencies per block
+// instead of 16; elements outside the triangle load the channel's first
+// entry (in range, unused) so that the loads stay branch-free.
+typedef float xf32x16 __attribute__((ext_vector_type(16)));
+template <int G>
+__device__ __forceinline__ void tri_block_out(float2 *__restrict__ C, double2 *__restrict__ acc, long cb, int NN,
+                                              int a0, int kh, int b, float scale, const xf32x16 &dre,
+                                              const xf32x16 &dim) {
+  static_assert(16 % G == 0, "groups must split the 16 registers");
+#pragma unroll
+  for (int g0 = 0; g0 < 16; g0 += G) {
+    long off[G];
+    bool ok[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+      const int r = g0 + i;
+      const int a = a0 + 8 * (r >> 2) + 4 * kh + (r & 3);
+      ok[i] = a < NN && b < NN && a >= b;
+      off[i] = cb + (ok[i] ? (long)a * (a + 1) / 2 + b : 0);
+    }
+    if (acc) {
+      double2 old[G];
+#pragma unroll
+      for (int i = 0; i < G; i++) old[i] = acc[off[i]];
+#pragma unroll
+      for (int i = 0; i < G; i++)
+        if (ok[i]) {
+          const float vr = scale * dre[g0 + i], vi = -scale * dim[g0 + i];
+          C[off[i]] = make_float2(vr, vi);
+          acc[off[i]] = make_double2(old[i].x + (double)vr, old[i].y + (double)vi);
+        }
+    } else {
+#pragma unroll
+      for (int i = 0; i < G; i++)
+        if (ok[i]) C[off[i]] = make_float2(scale * dre[g0 + i], -scale * dim[g0 + i]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 int xeng_launch(const XArgs &g, int layout, int out, hipStream_t st);
 int xeng_launch_f16split(const XArgs &g, hipStream_t st);
 int xeng_launch_f8(const XArgs &g, const float *power, hipStream_t st);

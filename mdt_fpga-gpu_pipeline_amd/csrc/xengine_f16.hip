@@ -86,7 +86,6 @@ __device__ __forceinline__ void stage(const Prefetch &p, char *lds) {
   }
 }
 
-__device__ __forceinline__ long tri_bl16(int a, int b) { return (long)a * (a + 1) / 2 + b; }
 
 __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float oscale) {
   __shared__ __attribute__((aligned(16))) char lds[2 * 2 * HPLANE];  // [op I|J][hi|lo][row][k]
@@ -186,28 +185,14 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
   if (!any) return;
 
   // D[i][j] = V[b = j][a = i]; V[a][b] = conj(D) * alpha / sc^2
-  double2 *acc = (double2 *)g.acc64;
 #pragma unroll
   for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       if (!act[t][u]) continue;
-      const int b = bj * HT + wj * 64 + u * 32 + l32;
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int a = bi * HT + wi * 64 + t * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
-        if (a < NN && b < NN && a >= b) {
-          const long o = (long)batch * g.Cbatch + tri_bl16(a, b);
-          const float vr = oscale * accR[t][u][r], vi = -oscale * accI[t][u][r];
-          g.Cbase[o] = make_float2(vr, vi);
-          if (acc) {
-            double2 x = acc[o];
-            x.x += (double)vr;
-            x.y += (double)vi;
-            acc[o] = x;
-          }
-        }
-      }
+      tri_block_out<8>(g.Cbase, (double2 *)g.acc64, (long)batch * g.Cbatch, NN, bi * HT + wi * 64 + t * 32, kh,
+                    bj * HT + wj * 64 + u * 32 + l32, oscale, accR[t][u], accI[t][u]);
+      __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -335,26 +320,10 @@ template <int NB>
 __device__ __forceinline__ void wc_epilogue(const XArgs &g, int batch, float oscale, const int (&ra)[NB],
                                             const int (&rb)[NB], int l32, int kh, const f32x16 (&accR)[NB],
                                             const f32x16 (&accI)[NB]) {
-  const int NN = g.m;
-  double2 *acc = (double2 *)g.acc64;
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    const int bcol = rb[b] * 32 + l32;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int arow = ra[b] * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
-      if (arow < NN && bcol < NN && arow >= bcol) {
-        const long o = (long)batch * g.Cbatch + tri_bl16(arow, bcol);
-        const float vr = oscale * accR[b][r], vi = -oscale * accI[b][r];
-        g.Cbase[o] = make_float2(vr, vi);
-        if (acc) {
-          double2 x = acc[o];
-          x.x += (double)vr;
-          x.y += (double)vi;
-          acc[o] = x;
-        }
-      }
-    }
+    tri_block_out<8>(g.Cbase, (double2 *)g.acc64, (long)batch * g.Cbatch, g.m, ra[b] * 32, kh, rb[b] * 32 + l32, oscale,
+                  accR[b], accI[b]);
     __builtin_amdgcn_sched_barrier(0);
   }
 }

@@ -115,7 +115,6 @@ __device__ __forceinline__ i32x8 ld_op(const char *p) {
   return r;
 }
 
-__device__ __forceinline__ long tri_bl8(int a, int b) { return (long)a * (a + 1) / 2 + b; }
 
 __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *power, float count) {
   __shared__ __attribute__((aligned(16))) char lds[2 * Q8PLANE];  // [op I|J][row][k]
@@ -202,28 +201,14 @@ __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *p
   }
   if (!any) return;
 
-  double2 *acc = (double2 *)g.acc64;
 #pragma unroll
   for (int t = 0; t < 2; t++)
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       if (!act[t][u]) continue;
-      const int b = bj * Q8T + wj * 64 + u * 32 + l32;
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int a = bi * Q8T + wi * 64 + t * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
-        if (a < NN && b < NN && a >= b) {
-          const long o = (long)batch * g.Cbatch + tri_bl8(a, b);
-          const float vr = oscale * accR[t][u][r], vi = -oscale * accI[t][u][r];
-          g.Cbase[o] = make_float2(vr, vi);
-          if (acc) {
-            double2 x = acc[o];
-            x.x += (double)vr;
-            x.y += (double)vi;
-            acc[o] = x;
-          }
-        }
-      }
+      tri_block_out<8>(g.Cbase, (double2 *)g.acc64, (long)batch * g.Cbatch, NN, bi * Q8T + wi * 64 + t * 32, kh,
+                    bj * Q8T + wj * 64 + u * 32 + l32, oscale, accR[t][u], accI[t][u]);
+      __builtin_amdgcn_sched_barrier(0);
     }
 }
 
