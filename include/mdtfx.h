@@ -201,8 +201,7 @@ int mdt_write_binary(const char *path, const void *vis, int layout, int dtype, i
 
 /* ---------------------------------------------------------------- native fused path
  * One dump: int8 (I,Q) samples [Ns+T-1][NN][N][2] -> fused PFB FIR + FFT +
- * corner turn -> channel-major X input [N][NN][Ns] complex FP32 (row and
- * channel pitches below) -> MFMA
+ * corner turn -> channel-major X input (layouts below) -> MFMA
  * X-engine over the lower triangle -> visibilities [N][NN(NN+1)/2] complex
  * FP32, baseline bl(a,b) = a(a+1)/2 + b for a >= b, V = (1/Ns) sum_s
  * x_a conj(x_b) (the alpha of MDT_OneStation_client.cu:683).  Optionally adds
@@ -213,12 +212,37 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
                        int Nspectra, int window_kind, int xprec);
 int mdt_fx_set_stream(mdt_fx_plan plan, hipStream_t stream);
 int mdt_fx_set_precision(mdt_fx_plan plan, int xprec);
-/* bytes of the channel-major X input buffer the caller must provide */
+/* bytes of the X input buffer the caller must provide (layout and precision
+ * as set when it is filled) */
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes);
-/* X input layout: sample (channel k, input e, spectrum s) sits at element
- * k*chan_pitch + e*row_pitch + s, row_pitch = Ns rounded up to even + row_pad,
- * chan_pitch = NN*row_pitch + chan_pad.  The padding (even, in samples) keeps
- * rows and channels off power-of-two strides; defaults 8 and 64. */
+/* X input layouts.  In all, sample (channel k, input e, spectrum s) sits at
+ * element (s / block) * block_pitch + (k / chan_group) * chan_pitch
+ *         + e * row_pitch + (k % chan_group) * block + s % block
+ * (mdt_fx_xinput_strides returns the five values).
+ * MDT_XIN_PAIRS: chan_group = 2, block = the spectra in 64 bytes (8 samples
+ * of 8 bytes, 16 of the FP8 path's 4 bytes), row_pitch = 2*block, chan_pitch
+ * = NN*row_pitch, block_pitch = ceil(N/2)*chan_pitch: a block of spectra of
+ * channels 2j and 2j+1 of one input is one 128-byte piece.
+ * MDT_XIN_BLOCKS: chan_group = 1, block = the spectra in 128 bytes (16 or
+ * 32 samples), row_pitch = block, chan_pitch = NN*block, block_pitch =
+ * N*chan_pitch.
+ * MDT_XIN_ROWS: chan_group = 1, block_pitch = 0 (block = 2^30 > s): element
+ * k*chan_pitch + e*row_pitch + s, row_pitch = Ns rounded up to even +
+ * row_pad, chan_pitch = NN*row_pitch + chan_pad (FP8 path: both rounded up
+ * to a multiple of 4).  The padding (even, in samples) keeps rows and
+ * channels off power-of-two strides; defaults 8 and 64.
+ * MDT_XIN_AUTO (default): per X-engine precision, the layout whose F-engine
+ * writes and X-engine reads were fastest: pairs for the FP16 split, blocks
+ * for FP8, rows for FP32. */
+#define MDT_XIN_AUTO (-1)
+#define MDT_XIN_PAIRS 0
+#define MDT_XIN_ROWS 1
+#define MDT_XIN_BLOCKS 2
+int mdt_fx_set_xinput_layout(mdt_fx_plan plan, int layout);
+int mdt_fx_xinput_strides(mdt_fx_plan plan, size_t *block, size_t *block_pitch, size_t *chan_group,
+                          size_t *chan_pitch, size_t *row_pitch);
+/* rows layout only: padding, and the two pitches (MDT_ERR_INVALID_VALUE in
+ * the other layouts) */
 int mdt_fx_set_xinput_padding(mdt_fx_plan plan, int row_pad, int chan_pad);
 int mdt_fx_xinput_pitch(mdt_fx_plan plan, size_t *row_pitch, size_t *chan_pitch);
 /* copies the plan's FP32 window [Ntaps][Nchannels] to a device buffer */

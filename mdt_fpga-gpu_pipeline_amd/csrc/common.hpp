@@ -41,6 +41,29 @@ int check_channels(int N);
 int check_taps(int T);
 int check_spectra(int Ns);
 
+// -------------------------------------------------- native X input geometry
+// Sample (channel k, input e, spectrum s) of the fused path's X input sits at
+// element chan(k) + at(e, s), in samples:
+//   chan(k)  = (k >> lgG) * cp + (k mod 2^lgG) * 2^lgB
+//   at(e, s) = (s >> lgB) * bp + e * rp + (s mod 2^lgB)
+// Rows layout (lgG = 0, lgB = 30, bp = 0): k*cp + e*rp + s.  Pairs layout
+// (lgG = 1): the two channels of a pair side by side per input, spectra in
+// blocks of B = 2^lgB (64 bytes of samples), rp = 2B, cp = NN*2B, bp =
+// ceil(N/2)*cp, so that B consecutive spectra of two adjacent channels of one
+// input are one 128-byte piece.
+struct XGeom {
+  long rp = 0, cp = 0, bp = 0;
+  int lgB = 30, lgG = 0;
+  __host__ __device__ long chan(int k) const {
+    return (long)(k >> lgG) * cp + ((long)(k & ((1 << lgG) - 1)) << lgB);
+  }
+  __host__ __device__ long at(int e, int s) const {
+    return (long)(s >> lgB) * bp + (long)e * rp + (s & ((1 << lgB) - 1));
+  }
+  // spectrum part of at() (for loads whose row part is added separately)
+  __host__ __device__ long spec(int s) const { return (long)(s >> lgB) * bp + (s & ((1 << lgB) - 1)); }
+};
+
 // ----------------------------------------------------------- complex helpers
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }

@@ -10,9 +10,9 @@
 // fma-accumulates them into the SB outputs that use the frame, in tap order
 // (bit-identical to mdt_pps_batch_i8).  Each of the SB spectra is then
 // transformed with the shared Stockham core (bit-identical to
-// mdt_fft_exec_c2c), and the SB x N results are corner-turned through LDS so
-// that each store instruction writes 64-byte runs of SB consecutive spectra of
-// one (channel, input) row of the channel-major output [N][NN][Ns].
+// mdt_fft_exec_c2c), and the SB x N results are corner-turned through LDS
+// into the X input (rows or channel-pair layout, XGeom in common.hpp): each
+// store instruction writes runs of consecutive spectra of one (channel, input).
 #include "calib.hpp"
 #include "fengine.hpp"
 #include "fft_core.hpp"
@@ -132,12 +132,12 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
     if (valid_e) {
 #pragma unroll
       for (int m = 0; m < P; m++)
-        a.out[(long)(t + m * TPB) * a.cp + (long)e * a.rp + s0] =
+        a.out[a.xg.chan(t + m * TPB) + a.xg.at(e, s0)] =
             a.split ? split_sample(acc[0][m], a.split_scale) : acc[0][m];
     }
   } else {
     constexpr int CH = SB / 2;  // 16-byte chunks per (k, e) row segment
-    const bool vec = ((a.rp & 1) == 0) && ((a.cp & 1) == 0);
+    const bool vec = ((a.xg.rp | a.xg.cp | a.xg.bp) & 1) == 0;  // 16-byte aligned pairs (B is even)
 #pragma unroll
     for (int m = 0; m < P; m++) {
       __syncthreads();
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
           const int tt = q / CH, pr = q % CH;
           const int k = tt + m * TPB;
           const int sl = 2 * pr;
-          float2 *dst = a.out + (long)k * a.cp + (long)e * a.rp + s0 + sl;
+          float2 *dst = a.out + a.xg.chan(k) + a.xg.at(e, s0 + sl);
           const float2 v0 = lds[tt * C::ROW + sl], v1 = lds[tt * C::ROW + sl + 1];
           if (vec && sl + 1 < scount) {
             *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
@@ -173,8 +173,11 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
 // transforms with its own exchange buffer (the twiddle table is shared), then
 // the whole workgroup corner-turns through LDS so that every (channel, input)
 // row gets 4Q consecutive samples in one piece: 64 B and more per row instead
-// of 32 B, which is what the scattered-row writes of the channel-major output
-// need to run near the HBM write rate.  Per thread: 128-VGPR budget, taps read
+// of 32 B.  In the channel-pair layout the rows of channels 2j and 2j+1 of an
+// input are adjacent within each block of spectra, so consecutive lanes
+// cover both and at N = 4096 every piece is 128 B (two channels x 8 spectra
+// of 8 bytes), which is what scattered writes need to run near the HBM write
+// rate.  Per thread: 128-VGPR budget, taps read
 // per item from a lane-contiguous copy (one 16-byte load per 4 taps), the four
 // spectra transformed one at a time in place in the FIR accumulators.  The
 // arithmetic and its order are those of fengine_kernel and of the drop-in
@@ -240,7 +243,11 @@ constexpr int fe_waves_per_simd() {
   return GAIN ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
 }
 
-template <int N, int T, int SPLIT, bool GAIN>
+// X input layout with its block and row pitch as constants: XL_ROWS, XL_PAIRS
+// (B = 64 bytes of samples, row pitch 2B, block pitch (N/2) cp) or XL_BLOCKS
+// (B = 128 bytes of samples, row pitch B, block pitch N cp)
+enum { XL_ROWS = 0, XL_PAIRS = 1, XL_BLOCKS = 2 };
+template <int N, int T, int SPLIT, bool GAIN, int XL>
 __global__ __launch_bounds__((PCfg<N, SPLIT, GAIN>::THREADS), (fe_waves_per_simd<N, SPLIT, GAIN>())) void
 fengine_persistent_kernel(FArgs a, long nitems) {
   using C = PCfg<N, SPLIT, GAIN>;
@@ -383,7 +390,10 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     // (channel) r of a round holds the item's SPAN spectra; each 16-byte
     // chunk of a row goes to one lane, the CPR chunks of a row to consecutive
     // lanes, so each store instruction writes whole SPAN-sample row pieces
-    const bool vec = SPLIT == 2 ? (((a.rp | a.cp) & 3) == 0) : (((a.rp | a.cp) & 1) == 0);
+    // whole 16-byte chunks: always in the pairs layout (cp = NN*2B, B and the
+    // chunk offsets multiples of 4 / 2 samples), rows when the pitches allow
+    const long pitches = a.xg.rp | a.xg.cp;
+    const bool vec = XL != XL_ROWS || (SPLIT == 2 ? ((pitches & 3) == 0) : ((pitches & 1) == 0));
     const int scnt_item = min(SPAN, a.Ns - s0);
 #pragma unroll
     for (int m0 = 0; m0 < P; m0 += MR) {
@@ -403,7 +413,20 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         const int row = c / CPR, cc = c % CPR;
         const int k = row % TPB + (m0 + row / TPB) * TPB;
         const int s1 = cc * SPC;  // first spectrum of the chunk, relative to s0
-        const long o = (long)k * a.cp + (long)e * a.rp + s0 + s1;
+        long o;  // xg.chan(k) + xg.at(e, s0 + s1)
+        if constexpr (XL == XL_ROWS) {
+          o = (long)k * a.xg.cp + (long)e * a.xg.rp + s0 + s1;
+        } else {
+          // s0 (uniform) is a multiple of SPAN, s1 < SPAN: keep the block
+          // index uniform where a block holds whole items
+          constexpr int LGB = (SPLIT == 2 ? 4 : 3) + (XL == XL_BLOCKS ? 1 : 0), B = 1 << LGB;
+          const int blk = SPAN >= B ? (s0 >> LGB) + (s1 >> LGB) : (s0 >> LGB);
+          const int in_blk = SPAN >= B ? (s1 & (B - 1)) : (s0 & (B - 1)) + s1;
+          if constexpr (XL == XL_PAIRS)
+            o = ((long)blk * (N / 2) + (k >> 1)) * a.xg.cp + ((e * 2 + (k & 1)) * B + in_blk);
+          else
+            o = ((long)blk * N + k) * a.xg.cp + (e * B + in_blk);
+        }
         const float2 *src_ = buf + row * ROWW + s1;
         if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
           unsigned *d4 = (unsigned *)a.out + o;
@@ -437,7 +460,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   }
 }
 
-template <int N, int T, int SPLIT, bool GAIN>
+template <int N, int T, int SPLIT, bool GAIN, int XL>
 static int launch_persistent_k(const FArgs &a, hipStream_t st) {
   using C = PCfg<N, SPLIT, GAIN>;
   static_assert((C::MR * C::TPB * C::CPR) % C::THREADS == 0, "store chunks must divide among the threads");
@@ -445,24 +468,40 @@ static int launch_persistent_k(const FArgs &a, hipStream_t st) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SPLIT, GAIN>,
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SPLIT, GAIN, XL>,
                                                    C::THREADS, 0) != hipSuccess)
     return fail(MDT_ERR_DEVICE, "device attributes for the F-engine grid");
   long grid = (long)cus * (per_cu > 0 ? per_cu : 1);
   if (grid > nitems) grid = nitems;
-  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SPLIT, GAIN>), dim3((unsigned)grid), dim3(C::THREADS), 0,
+  hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SPLIT, GAIN, XL>), dim3((unsigned)grid), dim3(C::THREADS), 0,
                      st, a, nitems);
   return check_launch("fengine_persistent_kernel");
+}
+
+template <int N, int T, int SPLIT>
+static int launch_persistent_s(const FArgs &a, hipStream_t st) {
+  const bool gain = a.gain != nullptr;
+  const XGeom &g = a.xg;
+  constexpr int LGB = SPLIT == 2 ? 4 : 3;  // pairs; blocks one more
+  if (g.lgG == 1 && g.lgB == LGB && g.rp == (2L << LGB) && g.bp == (long)(N / 2) * g.cp)
+    return gain ? launch_persistent_k<N, T, SPLIT, true, XL_PAIRS>(a, st)
+                : launch_persistent_k<N, T, SPLIT, false, XL_PAIRS>(a, st);
+  if (g.lgG == 0 && g.lgB == LGB + 1 && g.rp == (2L << LGB) && g.bp == (long)N * g.cp)
+    return gain ? launch_persistent_k<N, T, SPLIT, true, XL_BLOCKS>(a, st)
+                : launch_persistent_k<N, T, SPLIT, false, XL_BLOCKS>(a, st);
+  if (g.lgG == 0 && g.bp == 0)
+    return gain ? launch_persistent_k<N, T, SPLIT, true, XL_ROWS>(a, st)
+                : launch_persistent_k<N, T, SPLIT, false, XL_ROWS>(a, st);
+  return fail(MDT_ERR_INVALID_VALUE, "X input geometry not served by the persistent F-engine");
 }
 
 template <int N, int T>
 static int launch_persistent(const FArgs &a, hipStream_t st) {
   if (a.Wt == nullptr) return fail(MDT_ERR_INVALID_VALUE, "persistent F-engine needs the arranged taps");
-  const bool gain = a.gain != nullptr;
   switch (a.split) {
-    case 0: return gain ? launch_persistent_k<N, T, 0, true>(a, st) : launch_persistent_k<N, T, 0, false>(a, st);
-    case 1: return gain ? launch_persistent_k<N, T, 1, true>(a, st) : launch_persistent_k<N, T, 1, false>(a, st);
-    case 2: return gain ? launch_persistent_k<N, T, 2, true>(a, st) : launch_persistent_k<N, T, 2, false>(a, st);
+    case 0: return launch_persistent_s<N, T, 0>(a, st);
+    case 1: return launch_persistent_s<N, T, 1>(a, st);
+    case 2: return launch_persistent_s<N, T, 2>(a, st);
     default: return fail(MDT_ERR_INVALID_VALUE, "unknown F-engine output format %d", a.split);
   }
 }

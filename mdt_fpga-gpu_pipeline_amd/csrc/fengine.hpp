@@ -10,9 +10,9 @@ struct FArgs {
   const float *W;       // [T][N] FP32 window
   const float *Wt;      // persistent kernel: the window as [m][i/4][t][4] (arrange_taps)
   const float2 *tw;     // W_N^k twiddles
-  float2 *out;          // channel-major: (k, e, s) at k*cp + e*rp + s, complex FP32
+  float2 *out;          // X input: (k, e, s) at xg.chan(k) + xg.at(e, s) (samples)
   int NN, Ns, T;
-  long rp, cp;          // row pitch (>= Ns) and channel pitch (>= NN*rp), in samples
+  XGeom xg;             // X input geometry (rows or channel pairs)
   const float2 *gain;   // optional [NN] complex gains
   const float *delay;   // optional [NN] delays in samples
   int split;            // 0: FP32 (re, im); 1: FP16 (hi_re, hi_im, lo_re, lo_im) of x*split_scale;

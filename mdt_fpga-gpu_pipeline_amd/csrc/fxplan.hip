@@ -23,9 +23,10 @@ struct mdt_fx_plan_s {
   float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15
   float *power = nullptr;  // FP8 path: per-channel power from the F-engine
   float *Wt = nullptr;     // window arranged for the persistent F-engine (when it serves the shape)
-  int row_pad = 8, chan_pad = 64;  // X input padding (samples) after each row / channel
-  // pitches in samples; 16-byte aligned rows: even for 8-byte samples, a
-  // multiple of 4 for the 4-byte FP8-path samples
+  int layout = MDT_XIN_AUTO;
+  int row_pad = 8, chan_pad = 64;  // rows layout: padding (samples) after each row / channel
+  // rows-layout pitches in samples; 16-byte aligned rows: even for 8-byte
+  // samples, a multiple of 4 for the 4-byte FP8-path samples
   long rp() const {
     const long r = (long)Ns + (Ns & 1) + row_pad;
     return xprec == MDT_XPREC_FP8 ? (r + 3) & ~3L : r;
@@ -33,6 +34,41 @@ struct mdt_fx_plan_s {
   long cp() const {
     const long c = (long)NN * rp() + chan_pad;
     return xprec == MDT_XPREC_FP8 ? (c + 3) & ~3L : c;
+  }
+  // the layout in use: MDT_XIN_AUTO picks, per X-engine, the one it reads
+  // best with the F-engine's writes (measured at the operating points)
+  int xin_layout() const {
+    if (layout != MDT_XIN_AUTO) return layout;
+    return xprec == MDT_XPREC_FP32 ? MDT_XIN_ROWS : (xprec == MDT_XPREC_FP8 ? MDT_XIN_BLOCKS : MDT_XIN_PAIRS);
+  }
+  // the X input geometry of the current layout and precision (XGeom)
+  mdt::XGeom geom() const {
+    mdt::XGeom g;
+    const int lay = xin_layout();
+    const int lg_esize = xprec == MDT_XPREC_FP8 ? 2 : 3;
+    if (lay == MDT_XIN_ROWS) {
+      g.rp = rp();
+      g.cp = cp();
+    } else if (lay == MDT_XIN_PAIRS) {  // two channels x 64 bytes of spectra per piece
+      g.lgB = 6 - lg_esize;
+      g.lgG = 1;
+      g.rp = 2L << g.lgB;
+      g.cp = (long)NN * g.rp;
+      g.bp = (long)((N + 1) / 2) * g.cp;
+    } else {  // MDT_XIN_BLOCKS: one channel x 128 bytes of spectra per piece
+      g.lgB = 7 - lg_esize;
+      g.rp = 1L << g.lgB;
+      g.cp = (long)NN * g.rp;
+      g.bp = (long)N * g.cp;
+    }
+    return g;
+  }
+  // X input extent in samples
+  long xin_samples() const {
+    const mdt::XGeom g = geom();
+    if (xin_layout() == MDT_XIN_ROWS) return (long)N * g.cp;
+    const long B = 1L << g.lgB;
+    return (((long)Ns + B - 1) / B) * g.bp;
   }
 };
 
@@ -162,7 +198,32 @@ int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *de
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(bytes);
-  *bytes = elem_bytes(plan) * (size_t)plan->N * (size_t)plan->cp();
+  *bytes = elem_bytes(plan) * (size_t)plan->xin_samples();
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_set_xinput_layout(mdt_fx_plan plan, int layout) {
+  MDT_REQUIRE_PTR(plan);
+  if (layout != MDT_XIN_AUTO && layout != MDT_XIN_PAIRS && layout != MDT_XIN_ROWS && layout != MDT_XIN_BLOCKS)
+    return fail(MDT_ERR_INVALID_VALUE, "unknown X input layout %d", layout);
+  plan->layout = layout;
+  return MDT_SUCCESS;
+}
+
+int mdt_fx_xinput_strides(mdt_fx_plan plan, size_t *block, size_t *block_pitch, size_t *chan_group,
+                          size_t *chan_pitch, size_t *row_pitch) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(block);
+  MDT_REQUIRE_PTR(block_pitch);
+  MDT_REQUIRE_PTR(chan_group);
+  MDT_REQUIRE_PTR(chan_pitch);
+  MDT_REQUIRE_PTR(row_pitch);
+  const XGeom g = plan->geom();
+  *block = (size_t)1 << g.lgB;
+  *block_pitch = (size_t)g.bp;
+  *chan_group = (size_t)1 << g.lgG;
+  *chan_pitch = (size_t)g.cp;
+  *row_pitch = (size_t)g.rp;
   return MDT_SUCCESS;
 }
 
@@ -180,6 +241,8 @@ int mdt_fx_xinput_pitch(mdt_fx_plan plan, size_t *row_pitch, size_t *chan_pitch)
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(row_pitch);
   MDT_REQUIRE_PTR(chan_pitch);
+  if (plan->xin_layout() != MDT_XIN_ROWS)
+    return fail(MDT_ERR_INVALID_VALUE, "the X input is not in the rows layout (mdt_fx_xinput_strides)");
   *row_pitch = (size_t)plan->rp();
   *chan_pitch = (size_t)plan->cp();
   return MDT_SUCCESS;
@@ -208,8 +271,7 @@ int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInput
   a.NN = plan->NN;
   a.Ns = plan->Ns;
   a.T = plan->T;
-  a.rp = plan->rp();
-  a.cp = plan->cp();
+  a.xg = plan->geom();
   a.gain = plan->gain;
   a.delay = plan->delay;
   a.split = (plan->xprec == MDT_XPREC_FP16_SPLIT) ? 1 : (plan->xprec == MDT_XPREC_FP8 ? 2 : 0);
@@ -235,9 +297,13 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
   g.batch = plan->N;
   g.alpha = make_float2(1.0f / (float)plan->Ns, 0.f);  // MDT_OneStation_client.cu:683
   g.beta = make_float2(0.f, 0.f);
+  const XGeom xg = plan->geom();
   g.Abase = g.Bbase = (const float2 *)XInputNative;
-  g.Abatch = g.Bbatch = plan->cp();
-  g.ldI = g.ldJ = plan->rp();
+  g.Abatch = g.Bbatch = xg.cp;
+  g.ldI = g.ldJ = xg.rp;
+  g.xlgB = xg.lgB;
+  g.xlgG = xg.lgG;
+  g.xbp = xg.bp;
   g.Cbase = (float2 *)Vis;
   g.Cbatch = (long)NN * (NN + 1) / 2;
   g.herm = 1;

@@ -39,7 +39,7 @@ constexpr int XPLANE = 2 * XRL;  // floats per sample: re row + im row
 // zeroed, so all loads of a chunk are in flight together.
 template <int LAYOUT>
 __device__ __forceinline__ void load_chunk(float *lds, const float2 *__restrict__ base, long ld,
-                                           int row0, int nrows, int s0, int K) {
+                                           int row0, int nrows, int s0, int K, int lgB, long bp) {
   const int tid = threadIdx.x;
   if constexpr (LAYOUT == XL_KM) {
     const int r = tid & 63, sq = tid >> 6;  // 64 rows x 4 sample lanes
@@ -63,13 +63,13 @@ __device__ __forceinline__ void load_chunk(float *lds, const float2 *__restrict_
     const int sp = tid & 15, rq = tid >> 4;  // 16 sample pairs x 16 rows
     const int s = 2 * sp;
     const bool s_ok0 = s0 + s < K, s_ok1 = s0 + s + 1 < K;
-    const int sc = s_ok0 ? s0 + s : 0;
+    const long so = xin_spec(lgB, bp, s_ok0 ? s0 + s : 0);  // pairs (s, s+1) never straddle a block
     float4 v[XT / 16];
 #pragma unroll
     for (int q = 0; q < XT / 16; q++) {
       const int grow = row0 + rq + 16 * q;
       const bool r_ok = grow < nrows;
-      const float4 x = *(const float4 *)(base + (long)(r_ok ? grow : 0) * ld + sc);
+      const float4 x = *(const float4 *)(base + (long)(r_ok ? grow : 0) * ld + so);
       const bool ok0 = r_ok && s_ok0, ok1 = r_ok && s_ok1;
       v[q] = make_float4(ok0 ? x.x : 0.f, ok0 ? x.y : 0.f, ok1 ? x.z : 0.f, ok1 ? x.w : 0.f);
     }
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void xeng_f32_kernel(XArgs g) {
   }
   const float2 *baseI, *baseJ;
   if (OUT == XO_TRI) {
-    baseI = g.Abase + (long)batch * g.Abatch;
+    baseI = g.Abase + xin_batch(g, batch);
     baseJ = baseI;
   } else {
     baseJ = g.Atab ? g.Atab[batch] : g.Abase + (long)batch * g.Abatch;
@@ -153,8 +153,8 @@ __global__ __launch_bounds__(256) void xeng_f32_kernel(XArgs g) {
 
   for (int s0 = 0; s0 < g.k; s0 += XKC) {
     __syncthreads();
-    load_chunk<LAYOUT>(ldsI, baseI, g.ldI, bi * XT, nI, s0, g.k);
-    if (!same) load_chunk<LAYOUT>(ldsJ, baseJ, g.ldJ, bj * XT, nJ, s0, g.k);
+    load_chunk<LAYOUT>(ldsI, baseI, g.ldI, bi * XT, nI, s0, g.k, g.xlgB, g.xbp);
+    if (!same) load_chunk<LAYOUT>(ldsJ, baseJ, g.ldJ, bj * XT, nJ, s0, g.k, g.xlgB, g.xbp);
     __syncthreads();
     if (!skip) {
 #pragma unroll 8

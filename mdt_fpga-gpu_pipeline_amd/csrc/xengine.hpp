@@ -31,7 +31,21 @@ struct XArgs {
   double *acc64;  // XO_TRI only: FP64 complex accumulator, += V (nullable)
   int prec;       // MDT_XPREC_*
   float in_scale; // FP16 split: exact power-of-two pre-scale of the FP32 samples
+  // native X input geometry (XGeom, common.hpp) beyond the row pitch: batch
+  // b's operand starts at base + (b >> xlgG) * Abatch + (b mod 2^xlgG) *
+  // 2^xlgB, sample (row, s) at row * ld + (s >> xlgB) * xbp + (s mod
+  // 2^xlgB).  The defaults give base + b * Abatch + row * ld + s.
+  int xlgB = 30, xlgG = 0;
+  long xbp = 0;
 };
+
+// offset of batch b's operand, and the spectrum part of a sample's offset
+__device__ __forceinline__ long xin_batch(const XArgs &g, int b) {
+  return (long)(b >> g.xlgG) * g.Abatch + ((long)(b & ((1 << g.xlgG) - 1)) << g.xlgB);
+}
+__device__ __forceinline__ long xin_spec(int lgB, long bp, int s) {
+  return (long)(s >> lgB) * bp + (s & ((1 << lgB) - 1));
+}
 
 // Triangle epilogue of one 32x32 MFMA block: register r of the lane holds
 // the block element (a = a0 + 8(r/4) + 4 kh + r%4, b) whose value is

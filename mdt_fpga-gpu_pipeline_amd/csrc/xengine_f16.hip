@@ -51,17 +51,17 @@ struct Prefetch {
 // samples read a clamped, valid address and are zeroed afterwards.  Rows are
 // 16-byte aligned (even row pitch, even channel pitch).
 __device__ __forceinline__ void fetch(Prefetch &p, const float2 *__restrict__ base, long ld,
-                                      int row0, int nrows, int s0, int K) {
+                                      int row0, int nrows, int s0, int K, int lgB, long bp) {
   const int seg = threadIdx.x & 7, rq = threadIdx.x >> 3;
   const int s = s0 + 2 * seg;
   const bool s_ok0 = s < K, s_ok1 = s + 1 < K;
-  const int sc = s_ok0 ? s : 0;
+  const long so = xin_spec(lgB, bp, s_ok0 ? s : 0);  // pairs (s, s+1) never straddle a block
   unsigned ok = 0;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int row = row0 + rq + 32 * q;
     const bool r_ok = row < nrows;
-    p.v[q] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + sc);
+    p.v[q] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + so);
     ok |= ((r_ok && s_ok0) ? 1u : 0u) << (2 * q);
     ok |= ((r_ok && s_ok1) ? 1u : 0u) << (2 * q + 1);
   }
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
   const int Q = tile - P * (P + 1) / 2;
   const int bi = P, bj = Q;  // I rows <-> a, J rows <-> b, a >= b
   const bool same = (bi == bj);
-  const float2 *base = g.Abase + (long)batch * g.Abatch;
+  const float2 *base = g.Abase + xin_batch(g, batch);
   const int NN = g.m, K = g.k;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -132,8 +132,8 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
   char *ldsI = lds;
   char *ldsJ = same ? lds : lds + 2 * HPLANE;
   Prefetch pI, pJ;
-  fetch(pI, base, g.ldI, bi * HT, NN, 0, K);
-  if (!same) fetch(pJ, base, g.ldJ, bj * HT, NN, 0, K);
+  fetch(pI, base, g.ldI, bi * HT, NN, 0, K, g.xlgB, g.xbp);
+  if (!same) fetch(pJ, base, g.ldJ, bj * HT, NN, 0, K, g.xlgB, g.xbp);
 
   for (int s0 = 0; s0 < K; s0 += HKC) {
     __syncthreads();
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float os
     if (!same) stage(pJ, lds + 2 * HPLANE);
     __syncthreads();
     if (s0 + HKC < K) {  // prefetch the next chunk while this one is multiplied
-      fetch(pI, base, g.ldI, bi * HT, NN, s0 + HKC, K);
-      if (!same) fetch(pJ, base, g.ldJ, bj * HT, NN, s0 + HKC, K);
+      fetch(pI, base, g.ldI, bi * HT, NN, s0 + HKC, K, g.xlgB, g.xbp);
+      if (!same) fetch(pJ, base, g.ldJ, bj * HT, NN, s0 + HKC, K, g.xlgB, g.xbp);
     }
     if (any) {
 #pragma unroll
@@ -233,8 +233,8 @@ struct WcPrefetch {
   unsigned ok0, ok1;  // bit i: first / second sample of load i valid
 };
 
-__device__ __forceinline__ void wc_fetch(WcPrefetch &p, const float2 *__restrict__ base, long ld, int NN,
-                                         int s0, int K) {
+__device__ __forceinline__ void wc_fetch(WcPrefetch &p, const float2 *__restrict__ base, long ld, int lgB,
+                                         long bp, int NN, int s0, int K) {
   unsigned ok0 = 0, ok1 = 0;
 #pragma unroll
   for (int i = 0; i < WC_LOADS; i++) {
@@ -243,7 +243,7 @@ __device__ __forceinline__ void wc_fetch(WcPrefetch &p, const float2 *__restrict
     const int s = s0 + 2 * seg;
     const bool r_ok = row < NN;
     const int sc = s < K ? s : 0;
-    p.v[i] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + sc);
+    p.v[i] = *(const float4 *)(base + (long)(r_ok ? row : 0) * ld + xin_spec(lgB, bp, sc));
     ok0 |= ((r_ok && s < K) ? 1u : 0u) << i;
     ok1 |= ((r_ok && s + 1 < K) ? 1u : 0u) << i;
   }
@@ -331,23 +331,24 @@ __device__ __forceinline__ void wc_epilogue(const XArgs &g, int batch, float osc
 // the chunk loop of one wave kind: two-deep register prefetch (pa holds chunk
 // c+1, pb chunk c+2 while chunk c is multiplied out of LDS), LDS double buffer
 template <typename Chunk>
-__device__ __forceinline__ void wc_loop(char *lds, const float2 *base, long ld, int NN, int K, Chunk chunk) {
+__device__ __forceinline__ void wc_loop(char *lds, const float2 *base, long ld, int lgB, long bp, int NN, int K,
+                                        Chunk chunk) {
   const int nchunk = (K + WC_KC - 1) / WC_KC;
   WcPrefetch pa, pb;
-  wc_fetch(pa, base, ld, NN, 0, K);
+  wc_fetch(pa, base, ld, lgB, bp, NN, 0, K);
   wc_stage(pa, lds);
-  if (nchunk > 1) wc_fetch(pa, base, ld, NN, WC_KC, K);
-  if (nchunk > 2) wc_fetch(pb, base, ld, NN, 2 * WC_KC, K);
+  if (nchunk > 1) wc_fetch(pa, base, ld, lgB, bp, NN, WC_KC, K);
+  if (nchunk > 2) wc_fetch(pb, base, ld, lgB, bp, NN, 2 * WC_KC, K);
   for (int c = 0; c < nchunk; c += 2) {
     __syncthreads();  // chunk c staged in buffer 0; buffer 1 is free
     chunk(lds);
     if (c + 1 < nchunk) wc_stage(pa, lds + WC_BUF);
-    if (c + 3 < nchunk) wc_fetch(pa, base, ld, NN, (c + 3) * WC_KC, K);
+    if (c + 3 < nchunk) wc_fetch(pa, base, ld, lgB, bp, NN, (c + 3) * WC_KC, K);
     if (c + 1 < nchunk) {
       __syncthreads();  // chunk c+1 staged in buffer 1; buffer 0 is free
       chunk(lds + WC_BUF);
       if (c + 2 < nchunk) wc_stage(pb, lds);
-      if (c + 4 < nchunk) wc_fetch(pb, base, ld, NN, (c + 4) * WC_KC, K);
+      if (c + 4 < nchunk) wc_fetch(pb, base, ld, lgB, bp, NN, (c + 4) * WC_KC, K);
     }
   }
 }
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
   const long nwg = gridDim.x;
   const long w = blockIdx.x;
   const int batch = (int)((nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w);
-  const float2 *base = g.Abase + (long)batch * g.Abatch;
+  const float2 *base = g.Abase + xin_batch(g, batch);
   const int NN = g.m, K = g.k;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
       }
     }
     const int irow = r * 32 + l32;
-    wc_loop(lds, base, g.ldI, NN, K, [&](const char *bufp) { wc_chunk_a(bufp, irow, l32, kh, accR, accI); });
+    wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_a(bufp, irow, l32, kh, accR, accI); });
     wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
   } else {
     int ra[WC_B], rb[WC_B];
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
         accI[b][q] = 0.f;
       }
     }
-    wc_loop(lds, base, g.ldI, NN, K, [&](const char *bufp) { wc_chunk_b(bufp, ra, rb, l32, kh, accR, accI); });
+    wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_b(bufp, ra, rb, l32, kh, accR, accI); });
     wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
   }
 }

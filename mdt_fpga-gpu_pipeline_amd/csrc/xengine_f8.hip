@@ -53,16 +53,16 @@ struct Prefetch8 {
 
 // 128 rows x 32 samples of 4 bytes: 8 lanes x 16 bytes per row, branch-free
 __device__ __forceinline__ void fetch8(Prefetch8 &p, const unsigned *__restrict__ base, long ld,
-                                       int row0, int nrows, int s0, int K) {
+                                       int row0, int nrows, int s0, int K, int lgB, long bp) {
   const int seg = threadIdx.x & 7, rq = threadIdx.x >> 3;
   const int s = s0 + 4 * seg;
-  const int sc = s < K ? s : 0;
+  const long so = xin_spec(lgB, bp, s < K ? s : 0);  // 4 samples never straddle a block
   unsigned okm = 0;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int row = row0 + rq + 32 * q;
     const bool r_ok = row < nrows;
-    p.v[q] = *(const uint4 *)(base + (long)(r_ok ? row : 0) * ld + sc);
+    p.v[q] = *(const uint4 *)(base + (long)(r_ok ? row : 0) * ld + so);
 #pragma unroll
     for (int i = 0; i < 4; i++) okm |= ((r_ok && s + i < K) ? 1u : 0u) << (4 * q + i);
   }
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *p
   const int Q = tile - P * (P + 1) / 2;
   const int bi = P, bj = Q;
   const bool same = (bi == bj);
-  const unsigned *base = (const unsigned *)g.Abase + (long)batch * g.Abatch;
+  const unsigned *base = (const unsigned *)g.Abase + xin_batch(g, batch);
   const int NN = g.m, K = g.k;
 
   const int ec = chan_exponent(power[batch], count);
@@ -165,8 +165,8 @@ __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *p
   char *ldsI = lds;
   char *ldsJ = same ? lds : lds + Q8PLANE;
   Prefetch8 pI, pJ;
-  fetch8(pI, base, g.ldI, bi * Q8T, NN, 0, K);
-  if (!same) fetch8(pJ, base, g.ldJ, bj * Q8T, NN, 0, K);
+  fetch8(pI, base, g.ldI, bi * Q8T, NN, 0, K, g.xlgB, g.xbp);
+  if (!same) fetch8(pJ, base, g.ldJ, bj * Q8T, NN, 0, K, g.xlgB, g.xbp);
 
   for (int s0 = 0; s0 < K; s0 += Q8KC) {
     __syncthreads();
@@ -174,8 +174,8 @@ __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *p
     if (!same) stage8(pJ, lds + Q8PLANE, inv_scale);
     __syncthreads();
     if (s0 + Q8KC < K) {
-      fetch8(pI, base, g.ldI, bi * Q8T, NN, s0 + Q8KC, K);
-      if (!same) fetch8(pJ, base, g.ldJ, bj * Q8T, NN, s0 + Q8KC, K);
+      fetch8(pI, base, g.ldI, bi * Q8T, NN, s0 + Q8KC, K, g.xlgB, g.xbp);
+      if (!same) fetch8(pJ, base, g.ldJ, bj * Q8T, NN, s0 + Q8KC, K, g.xlgB, g.xbp);
     }
     if (any) {
       const int koff = kh * 32;  // 32 bytes = 16 samples per half-wave

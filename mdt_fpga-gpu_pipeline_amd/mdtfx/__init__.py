@@ -45,6 +45,10 @@ XPREC_FP16_SPLIT = 1
 XPREC_FP8 = 2
 VIS_FULL_COLMAJOR = 0
 VIS_TRIANGLE = 1
+XIN_AUTO = -1   # X input layout per X-engine precision (default)
+XIN_PAIRS = 0   # X input: channel pairs, spectra in 64-byte blocks
+XIN_ROWS = 1    # X input: channel-major rows with padding
+XIN_BLOCKS = 2  # X input: one channel, spectra in 128-byte blocks
 
 _lib = None
 
@@ -101,6 +105,8 @@ _SIGS = {
     "mdt_fx_set_precision": [_VP, _I],
     "mdt_fx_xinput_bytes": [_VP, ctypes.POINTER(_SZ)],
     "mdt_fx_set_xinput_padding": [_VP, _I, _I],
+    "mdt_fx_set_xinput_layout": [_VP, _I],
+    "mdt_fx_xinput_strides": [_VP] + [ctypes.POINTER(_SZ)] * 5,
     "mdt_fx_xinput_pitch": [_VP, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)],
     "mdt_fx_get_window": [_VP, _VP],
     "mdt_fx_fengine": [_VP, _VP, _VP],
@@ -373,30 +379,58 @@ class FxPlan:
         check(lib().mdt_fx_get_window(self._h, _ptr(W, torch.float32)))
         return W
 
+    def set_xinput_layout(self, layout):
+        """XIN_AUTO (default), XIN_PAIRS, XIN_BLOCKS or XIN_ROWS, for the
+        buffers filled from now on"""
+        check(lib().mdt_fx_set_xinput_layout(self._h, layout))
+
     def set_xinput_padding(self, row_pad, chan_pad):
+        """rows layout: padding (even, samples) after each row / channel"""
         check(lib().mdt_fx_set_xinput_padding(self._h, row_pad, chan_pad))
 
     def xinput_pitch(self):
+        """rows layout: (row_pitch, chan_pitch) in samples"""
         rp, cp = ctypes.c_size_t(), ctypes.c_size_t()
         check(lib().mdt_fx_xinput_pitch(self._h, ctypes.byref(rp), ctypes.byref(cp)))
         return rp.value, cp.value
 
+    def xinput_strides(self):
+        """(block, block_pitch, chan_group, chan_pitch, row_pitch): sample
+        (k, e, s) at (s // block) * block_pitch + (k // chan_group) * chan_pitch
+        + e * row_pitch + (k % chan_group) * block + s % block"""
+        v = [ctypes.c_size_t() for _ in range(5)]
+        check(lib().mdt_fx_xinput_strides(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def xinput_bytes(self):
+        nbytes = ctypes.c_size_t()
+        check(lib().mdt_fx_xinput_bytes(self._h, ctypes.byref(nbytes)))
+        return nbytes.value
+
     def alloc_xinput(self):
-        """X input buffer; returns the padded storage (contiguous, what the
-        library is given).  Use xinput_view() for the [N][NN][Ns] view."""
-        rp, cp = self.xinput_pitch()
+        """X input buffer (flat, mdt_fx_xinput_bytes); xinput_view() gives
+        the samples as [N][NN][Ns]."""
+        nbytes = self.xinput_bytes()
         if self.xprec == XPREC_FP8:  # 4-byte samples: FP16 (re, im)
-            return torch.empty((self.N, cp), dtype=torch.int32, device="cuda")
-        return torch.empty((self.N, cp), dtype=torch.complex64, device="cuda")
+            return torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
+        return torch.empty(nbytes // 8, dtype=torch.complex64, device="cuda")
 
     def xinput_view(self, buf):
-        """[N][NN][Ns] view: complex64 samples, or for FP8 plans float16
-        [N][NN][Ns][2] (re, im) of x*s."""
-        rp, cp = self.xinput_pitch()
-        if self.xprec == XPREC_FP8:
-            return torch.as_strided(buf.view(torch.float16), (self.N, self.NN, self.Ns, 2),
-                                    (2 * cp, 2 * rp, 2, 1))
-        return torch.as_strided(buf, (self.N, self.NN, self.Ns), (cp, rp, 1))
+        """The samples as [N][NN][Ns] complex64, or for FP8 plans float16
+        [N][NN][Ns][2] (re, im) of x*s: a view of buf in the rows layout, a
+        copy in the others."""
+        B, bp, G, cp, rp = self.xinput_strides()
+        f8 = self.xprec == XPREC_FP8
+        src = buf.view(torch.float16) if f8 else buf
+        u = 2 if f8 else 1   # tensor elements per sample
+        tail, tstr = ((2,), (1,)) if f8 else ((), ())
+        if G == 1 and bp == 0:
+            return torch.as_strided(src, (self.N, self.NN, self.Ns) + tail, (u * cp, u * rp, u) + tstr)
+        nblk, npair = -(-self.Ns // B), -(-self.N // G)
+        v = torch.as_strided(src, (nblk, npair, self.NN, G, B) + tail,
+                             (u * bp, u * cp, u * rp, u * B, u) + tstr)
+        v = v.permute(1, 3, 2, 0, 4, *range(5, 5 + len(tail)))
+        return v.reshape((npair * G, self.NN, nblk * B) + tail)[:self.N, :, :self.Ns]
 
     def alloc_vis(self):
         return torch.empty((self.N, self.nb), dtype=torch.complex64, device="cuda")

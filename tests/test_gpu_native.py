@@ -63,22 +63,77 @@ def test_native_chain_40in_hamming_vs_fp64():
     assert _tri_autonorm_err(v32, v64, NN) < 1e-5
 
 
-@pytest.mark.parametrize("pads", [(0, 0), (8, 64), (2, 2)])
-def test_xinput_padding_layout(pads):
-    """padded X input: same samples at k*chan_pitch + e*row_pitch + s, same vis"""
-    NN, N, T, Ns = 6, 512, 8, 12
+@pytest.mark.parametrize("xprec", [mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT, mdtfx.XPREC_FP8])
+def test_xinput_layouts_agree(xprec):
+    """every X input layout (rows with any padding, channel pairs, blocks, the
+    per-precision default) holds the same samples and gives the same
+    visibilities, bit for bit"""
+    NN, N, T, Ns = 6, 512, 8, 40
     x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=4, sigma=14.0, rho=0.2)
-    p0 = mdtfx.FxPlan(NN, N, T, Ns)
-    ref = _host(p0.xinput_view(p0.fengine(x8, p0.alloc_xinput())))
-    v0 = _host(p0.xengine(p0.fengine(x8, p0.alloc_xinput()), p0.alloc_vis()))
-    p = mdtfx.FxPlan(NN, N, T, Ns)
-    p.set_xinput_padding(*pads)
-    rp, cp = p.xinput_pitch()
-    assert rp == Ns + (Ns & 1) + pads[0] and cp == NN * rp + pads[1]
-    buf = p.fengine(x8, p.alloc_xinput())
-    assert np.array_equal(_host(p.xinput_view(buf)).view(np.uint32), ref.view(np.uint32))
-    v = _host(p.xengine(buf, p.alloc_vis()))
-    assert np.array_equal(v.view(np.uint32), v0.view(np.uint32))
+    ref = vref = eref = None
+    q = 4 if xprec == mdtfx.XPREC_FP8 else 1  # FP8 path: rows pitches rounded up to multiples of 4
+    for layout, pads in [(mdtfx.XIN_ROWS, (8, 64)), (mdtfx.XIN_ROWS, (0, 0)), (mdtfx.XIN_ROWS, (2, 2)),
+                         (mdtfx.XIN_PAIRS, None), (mdtfx.XIN_BLOCKS, None), (mdtfx.XIN_AUTO, None)]:
+        print(f"xinput layout={layout} pads={pads} xprec={xprec}", flush=True)
+        p = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, xprec)
+        p.set_xinput_layout(layout)
+        if pads is not None:
+            p.set_xinput_padding(*pads)
+            rp, cp = p.xinput_pitch()
+            assert rp == -(-(Ns + (Ns & 1) + pads[0]) // q) * q and cp == -(-(NN * rp + pads[1]) // q) * q
+            assert p.xinput_strides() == (1 << 30, 0, 1, cp, rp)
+        elif layout != mdtfx.XIN_AUTO:
+            with pytest.raises(mdtfx.MdtError):
+                p.xinput_pitch()  # rows layout only
+        buf = p.fengine(x8, p.alloc_xinput())
+        got = _host(p.xinput_view(buf)).view(np.uint16)
+        v = _host(p.xengine(buf, p.alloc_vis())).view(np.uint32)
+        e = _host(p.channel_exponents()) if xprec == mdtfx.XPREC_FP8 else None
+        if ref is None:
+            ref, vref, eref = got, v, e
+            continue
+        assert np.array_equal(got, ref)
+        if e is not None:  # the FP8 scaling follows the channel power, summed in any order
+            assert np.array_equal(e, eref)
+        assert np.array_equal(v, vref)
+
+
+@pytest.mark.parametrize("N,T", [(1024, 4), (4096, 8)])
+@pytest.mark.parametrize("xprec", [mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT, mdtfx.XPREC_FP8])
+@pytest.mark.parametrize("layout", [mdtfx.XIN_PAIRS, mdtfx.XIN_BLOCKS])
+def test_xinput_block_geometry(layout, xprec, N, T):
+    """channel pairs: sample (k, e, s) at (s // B) * block_pitch + (k // 2) *
+    chan_pitch + e * 2B + (k % 2) * B + s % B, B = 64 bytes of samples;
+    blocks: at (s // B) * block_pitch + k * chan_pitch + e * B + s % B, B =
+    128 bytes of samples.  The F-engine writes every sample where the formula
+    puts it and nothing past Ns (checked on the raw buffer)."""
+    NN, Ns = 5, 37
+    print(f"xinput geometry layout={layout} xprec={xprec} N={N}", flush=True)
+    esize = 4 if xprec == mdtfx.XPREC_FP8 else 8
+    plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, xprec)
+    plan.set_xinput_layout(layout)
+    B, bp, G, cp, rp = plan.xinput_strides()
+    if layout == mdtfx.XIN_PAIRS:
+        assert (B, G, rp, cp, bp) == (64 // esize, 2, 128 // esize, NN * 128 // esize, (N // 2) * NN * 128 // esize)
+    else:
+        assert (B, G, rp, cp, bp) == (128 // esize, 1, 128 // esize, NN * 128 // esize, N * NN * 128 // esize)
+    assert plan.xinput_bytes() == -(-Ns // B) * bp * esize
+    x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=9, sigma=12.0)
+    buf = plan.alloc_xinput()
+    buf.view(torch.uint8).fill_(0xA5)
+    plan.fengine(x8, buf)
+    raw = _host(buf).view(np.uint32 if esize == 4 else np.uint64)
+    rows = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, xprec)
+    rows.set_xinput_layout(mdtfx.XIN_ROWS)
+    want = np.ascontiguousarray(_host(rows.xinput_view(rows.fengine(x8, rows.alloc_xinput()))))
+    want = want.view(np.uint32 if esize == 4 else np.uint64).reshape(N, NN, Ns)
+    k, e, s = np.meshgrid(np.arange(N), np.arange(NN), np.arange(Ns), indexing="ij")
+    off = (s // B) * bp + (k // G) * cp + e * rp + (k % G) * B + s % B
+    assert np.array_equal(raw[off], want)
+    filler = np.uint32(0xA5A5A5A5) if esize == 4 else np.uint64(0xA5A5A5A5A5A5A5A5)
+    tail = np.ones(raw.shape, bool)
+    tail[off.ravel()] = False
+    assert np.all(raw[tail] == filler)
 
 
 @pytest.mark.parametrize("NN,N,T,Ns", [(3, 32, 4, 2), (65, 256, 4, 70), (130, 128, 8, 40)])
@@ -210,9 +265,12 @@ def test_front_end_refuses_undersized_buffers():
     NN, N, T, Ns = 4, 256, 4, 8
     plan = mdtfx.FxPlan(NN, N, T, Ns)
     x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=1)
-    small = torch.empty((N, NN * Ns), dtype=torch.complex64, device="cuda")  # unpadded
+    small = torch.empty(plan.xinput_bytes() // 8 - 1, dtype=torch.complex64, device="cuda")  # one sample short
     with pytest.raises(ValueError):
         plan.fengine(x8, small)
+    plan.set_xinput_layout(mdtfx.XIN_ROWS)  # rows: padded, larger than N x NN x Ns
+    with pytest.raises(ValueError):
+        plan.fengine(x8, torch.empty((N, NN * Ns), dtype=torch.complex64, device="cuda"))
     with pytest.raises(ValueError):
         plan.fengine(x8[:-1], plan.alloc_xinput())
     with pytest.raises(ValueError):
