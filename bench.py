@@ -275,9 +275,14 @@ def main():
     log("bench: FP16-split path")
     w16, f16_ms, x16_ms = time_path(torch, mdtfx, plan, x8, xin, vis16, acc, args.steps, args.warmup)
     plan.set_precision(mdtfx.XPREC_FP32)
+    del xin  # each precision's own X-input layout (FP32: padded rows, a little larger)
+    torch.cuda.empty_cache()
+    xin = plan.alloc_xinput()
     log("bench: FP32-MFMA path")
     w32, f32_ms, x32_ms = time_path(torch, mdtfx, plan, x8, xin, vis32, acc, args.steps, args.warmup)
     plan.set_precision(mdtfx.XPREC_FP16_SPLIT)
+    del xin
+    torch.cuda.empty_cache()
     torch.cuda.synchronize()
 
     per_dump = NS * N
@@ -298,7 +303,7 @@ def main():
     traffic = load_traffic()
     if f16_ms >= x16_ms:
         ach = f_bytes / (f16_ms * 1e-3) / 1e9
-        roof = {"kernel": "fengine_persistent_kernel<4096, 8, 1, false>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+        roof = {"kernel": "fengine_persistent_kernel<4096, 8, 1, false, 1>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get("fengine")}
     else:
         ach = x_flops / (x16_ms * 1e-3) / 1e12

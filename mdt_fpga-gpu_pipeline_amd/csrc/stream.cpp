@@ -46,6 +46,7 @@ struct mdt_stream_s {
   int8_t *in[kSlots] = {};  // (T-1 + Ns) spectra each
   int8_t *prime = nullptr;  // T-1 spectra: history of the first dump
   void *xin = nullptr;
+  size_t xin_bytes = 0;  // the X input allocated for the plan as it was at creation
   mdt_complex *vis[kSlots] = {};
   Ev ev[kSlots];
   bool busy[kSlots] = {};
@@ -111,6 +112,7 @@ int mdt_stream_create(mdt_stream *stream, mdt_fx_plan plan) {
     ok(hipStreamCreateWithFlags(&s->comp, hipStreamNonBlocking), "compute stream");
     ok(hipStreamCreateWithFlags(&s->cout, hipStreamNonBlocking), "copy-out stream");
     ok(hipMalloc(&s->xin, xbytes), "X input");
+    s->xin_bytes = xbytes;
     ok(hipMalloc(&s->prime, hist > 0 ? hist : 16), "history buffer");
     if (st == MDT_SUCCESS && hist > 0) ok(hipMemset(s->prime, 0, hist), "history zeroing");
     for (int b = 0; b < kSlots; b++) {
@@ -165,6 +167,11 @@ int mdt_stream_prime(mdt_stream stream, const int8_t *history) {
 
 // the enqueue sequence of one dump (see the top of this file)
 static int enqueue(mdt_stream s, int slot, double *Acc64) {
+  size_t xbytes = 0;  // the plan's layout or precision may have changed since creation
+  MDT_TRY(mdt_fx_xinput_bytes(s->plan, &xbytes));
+  if (xbytes > s->xin_bytes)
+    return fail(MDT_ERR_INVALID_VALUE, "the plan's X input (%zu bytes) outgrew the stream's (%zu bytes)", xbytes,
+                s->xin_bytes);
   Ev &e = s->ev[slot];
   const size_t hist = s->spectrum * (s->T - 1), stack = s->spectrum * s->Ns;
   const int8_t *src = s->prev < 0 ? s->prime : s->in[s->prev] + stack;  // previous dump's last T-1 spectra

@@ -100,3 +100,23 @@ def test_stream_slot_errors():
         st.stack(0)
     with pytest.raises(mdtfx.MdtError):  # Ns < T-1
         mdtfx.FxStream(mdtfx.FxPlan(16, 256, 8, 4))
+
+
+def test_stream_refuses_a_grown_x_input():
+    """the stream's X input is sized for the plan at creation: a later layout
+    whose X input is larger is refused before anything is queued"""
+    NN, N, T, Ns = 16, 256, 4, 32
+    x, xh = _signal(NN, N, T, Ns, 1, seed=5)
+    plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)  # pairs: no padding
+    st = mdtfx.FxStream(plan)
+    st.prime(xh[:T - 1])
+    st.stack(0)[...] = xh[T - 1:]
+    plan.set_xinput_layout(mdtfx.XIN_ROWS)  # padded rows: larger
+    with pytest.raises(mdtfx.MdtError):
+        st.submit(0)
+    plan.set_xinput_layout(mdtfx.XIN_AUTO)
+    st.submit(0)
+    st.wait(0)
+    got = st.vis(0).copy()
+    st.destroy()
+    assert np.array_equal(got, _per_dump(NN, N, T, Ns, mdtfx.XPREC_FP16_SPLIT, x, 0))
