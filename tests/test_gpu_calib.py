@@ -127,3 +127,40 @@ def test_loss_zero_at_truth(c5):
     loss, _, _ = c5["cal"].loss_grad(c5["V"], c5["M"], torch.from_numpy(c5["g_true"]).cuda(),
                                      torch.from_numpy(c5["tau_true"]).cuda(), want_grad=False)
     assert float(_host(loss)) == 0.0
+
+
+@pytest.mark.parametrize("N_,T_", [(1024, 4), (4096, 8)])
+@pytest.mark.parametrize("xprec", [mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT, mdtfx.XPREC_FP8])
+def test_gain_stage_in_every_xinput_layout(xprec, N_, T_):
+    """the F-engine's gain/delay stage writes the same samples in every X
+    input layout (its own compiled variant per layout), and the X-engine
+    gives the same visibilities"""
+    NN_, NS_ = 24, 40
+    print(f"gain stage layouts xprec={xprec} N={N_}", flush=True)
+    rng = np.random.default_rng(7)
+    g = torch.from_numpy((rng.uniform(0.5, 1.5, NN_) * np.exp(1j * rng.uniform(-3, 3, NN_))).astype(np.complex64)).cuda()
+    tau = torch.from_numpy(rng.uniform(-6, 6, NN_).astype(np.float32)).cuda()
+    x8 = mdtfx.synth_i8(NN_, N_, NS_ + T_ - 1, seed=8, sigma=14.0, rho=0.3, max_delay=4)
+    ref = vref = None
+    for layout in (mdtfx.XIN_ROWS, mdtfx.XIN_PAIRS, mdtfx.XIN_BLOCKS):
+        p = mdtfx.FxPlan(NN_, N_, T_, NS_, mdtfx.WINDOW_BH4, xprec)
+        p.set_xinput_layout(layout)
+        p.set_gains(g, tau)
+        buf = p.fengine(x8, p.alloc_xinput())
+        got = _host(p.xinput_view(buf)).view(np.uint16)
+        v = _host(p.xengine(buf, p.alloc_vis())).view(np.uint32)
+        # FP8: per-channel exponents from power sums added in any order; a
+        # channel whose exponent rounds the other way is compared no further
+        e = _host(p.channel_exponents()) if xprec == mdtfx.XPREC_FP8 else np.zeros(N_, np.int32)
+        if ref is None:
+            ref, vref, eref = got, v, e
+            # the gains are applied: the first input's samples differ from a plan without them
+            p0 = mdtfx.FxPlan(NN_, N_, T_, NS_, mdtfx.WINDOW_BH4, xprec)
+            p0.set_xinput_layout(layout)
+            plain = _host(p0.xinput_view(p0.fengine(x8, p0.alloc_xinput()))).view(np.uint16)
+            assert not np.array_equal(plain, got)
+            continue
+        assert np.array_equal(got, ref), layout
+        same = e == eref
+        assert same.mean() > 0.99
+        assert np.array_equal(v[same], vref[same]), layout
