@@ -28,10 +28,13 @@ def _tri_autonorm_err(vis, v64, NN):
 @pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (3, 32, 4, 11), (5, 64, 8, 17),
                                        (4, 2048, 4, 9), (3, 4096, 8, 16), (3, 8192, 4, 5),
                                        (3, 16384, 4, 3), (6, 256, 5, 12)])
-def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns):
-    """fused PFB+FFT+corner turn == pps_batch_i8 -> fft -> reorder, bit for bit"""
-    print(f"fengine NN={NN} N={N} T={T} Ns={Ns}", flush=True)
+@pytest.mark.parametrize("layout", [mdtfx.XIN_ROWS, mdtfx.XIN_PAIRS, mdtfx.XIN_BLOCKS])
+def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns, layout):
+    """fused PFB+FFT+corner turn == pps_batch_i8 -> fft -> reorder, bit for
+    bit, in every X input layout (persistent and generic F-engine kernels)"""
+    print(f"fengine NN={NN} N={N} T={T} Ns={Ns} layout={layout}", flush=True)
     plan = mdtfx.FxPlan(NN, N, T, Ns, mdtfx.WINDOW_HAMMING)
+    plan.set_xinput_layout(layout)
     x8 = mdtfx.synth_i8(NN, N, Ns + T - 1, seed=N + T, sigma=16.0, tones=((0.1234, 20.0),))
     xin = plan.xinput_view(plan.fengine(x8, plan.alloc_xinput()))
     W = plan.window()
