@@ -330,8 +330,9 @@ def unpack_u32(words, out=None):
 
 # ------------------------------------------------------------------ native path
 class FxPlan:
-    """One-dump plan of the fused path: int8 [Ns+T-1][NN][N][2] ->
-    X input [N][NN][Ns] complex64 -> triangle visibilities [N][NN(NN+1)/2]."""
+    """One-dump plan of the fused path: int8 [Ns+T-1][NN][N][2] -> X input
+    (channel pairs, spectrum blocks or padded rows: xinput_strides /
+    xinput_view) -> triangle visibilities [N][NN(NN+1)/2]."""
 
     def __init__(self, NN, N, T, Ns, window=WINDOW_BH4, xprec=XPREC_FP32):
         self._h = ctypes.c_void_p()
