@@ -231,9 +231,9 @@ int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes);
  * row_pad, chan_pitch = NN*row_pitch + chan_pad (FP8 path: both rounded up
  * to a multiple of 4).  The padding (even, in samples) keeps rows and
  * channels off power-of-two strides; defaults 8 and 64.
- * MDT_XIN_AUTO (default): per X-engine precision, the layout whose F-engine
- * writes and X-engine reads were fastest: pairs for the FP16 split, blocks
- * for FP8, rows for FP32. */
+ * MDT_XIN_AUTO (default): the layout whose F-engine writes and X-engine
+ * reads were fastest: pairs for the FP16 split up to 256 inputs, blocks for
+ * the FP16 split above 256 inputs and for FP8, rows for FP32. */
 #define MDT_XIN_AUTO (-1)
 #define MDT_XIN_PAIRS 0
 #define MDT_XIN_ROWS 1

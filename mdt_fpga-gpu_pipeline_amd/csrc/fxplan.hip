@@ -36,10 +36,15 @@ struct mdt_fx_plan_s {
     return xprec == MDT_XPREC_FP8 ? (c + 3) & ~3L : c;
   }
   // the layout in use: MDT_XIN_AUTO picks, per X-engine, the one it reads
-  // best with the F-engine's writes (measured at the operating points)
+  // best with the F-engine's writes (measured): pairs only where each
+  // channel's samples are read once (FP16 split, NN <= 256: one tile or the
+  // whole-channel kernel); the tiled X-engines that re-read a channel do
+  // not reuse the partner channel's half of each 128-byte line
   int xin_layout() const {
     if (layout != MDT_XIN_AUTO) return layout;
-    return xprec == MDT_XPREC_FP32 ? MDT_XIN_ROWS : (xprec == MDT_XPREC_FP8 ? MDT_XIN_BLOCKS : MDT_XIN_PAIRS);
+    if (xprec == MDT_XPREC_FP32) return MDT_XIN_ROWS;
+    if (xprec == MDT_XPREC_FP16_SPLIT && NN <= 256) return MDT_XIN_PAIRS;
+    return MDT_XIN_BLOCKS;
   }
   // the X input geometry of the current layout and precision (XGeom)
   mdt::XGeom geom() const {

@@ -103,6 +103,19 @@ def test_xinput_layouts_agree(xprec):
         assert np.array_equal(v[same], vref[same])
 
 
+@pytest.mark.parametrize("xprec,NN,G,B", [(mdtfx.XPREC_FP32, 256, 1, 1 << 30), (mdtfx.XPREC_FP16_SPLIT, 256, 2, 8),
+                                          (mdtfx.XPREC_FP16_SPLIT, 64, 2, 8), (mdtfx.XPREC_FP16_SPLIT, 300, 1, 16),
+                                          (mdtfx.XPREC_FP8, 512, 1, 32), (mdtfx.XPREC_FP8, 64, 1, 32)])
+def test_xinput_auto_layout(xprec, NN, G, B):
+    """the default layout: rows for FP32, channel pairs for the FP16 split up
+    to 256 inputs, 128-byte spectrum blocks above that and for FP8"""
+    plan = mdtfx.FxPlan(NN, 1024, 4, 64, mdtfx.WINDOW_BH4, xprec)
+    Bq, bp, Gq, cp, rp = plan.xinput_strides()
+    assert (Gq, Bq) == (G, B)
+    plan.set_precision(mdtfx.XPREC_FP32)  # the default follows the precision
+    assert plan.xinput_strides()[2] == 1 and plan.xinput_strides()[1] == 0
+
+
 @pytest.mark.parametrize("N,T", [(1024, 4), (4096, 8)])
 @pytest.mark.parametrize("xprec", [mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT, mdtfx.XPREC_FP8])
 @pytest.mark.parametrize("layout", [mdtfx.XIN_PAIRS, mdtfx.XIN_BLOCKS])
