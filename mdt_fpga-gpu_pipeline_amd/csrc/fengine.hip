@@ -243,6 +243,22 @@ constexpr int fe_waves_per_simd() {
   return GAIN ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
 }
 
+// Code-generation choices per output format, each the faster one measured
+// (C3 F-engine: FP16 hi/lo and FP32 outputs at N = 4096; C4: the FP8 path's
+// output at N = 1024).  At the 128-VGPR budget the register allocation
+// decides between them:
+//  pin:   each transform's outputs are materialised right after it, so the
+//         compiler does not sink the last butterflies into the store rounds
+//         (FP16 hi/lo 8.62 against 8.85 ms; slower for the other outputs);
+//  lanes: each lane's chunk offsets are computed once per item instead of
+//         in every store round (40 % fewer integer instructions; FP16 hi/lo
+//         with pin, FP8 12.9-13.1 against 13.1 ms; FP32 10.2 against 9.8 ms).
+template <int SPLIT>
+struct FeTuning {
+  static constexpr bool pin = SPLIT == 1;
+  static constexpr bool lanes = SPLIT != 0;
+};
+
 // X input layout with its block and row pitch as constants: XL_ROWS, XL_PAIRS
 // (B = 64 bytes of samples, row pitch 2B, block pitch (N/2) cp) or XL_BLOCKS
 // (B = 128 bytes of samples, row pitch B, block pitch N cp)
@@ -350,6 +366,10 @@ fengine_persistent_kernel(FArgs a, long nitems) {
 #pragma unroll
     for (int sl = 0; sl < SB; sl++) {
       fft_block_tw<N, P, false>(acc[sl], opaque(t), fbuf, tws);
+      if constexpr (FeTuning<SPLIT>::pin) {
+#pragma unroll
+        for (int m = 0; m < P; m++) asm volatile("" : "+v"(acc[sl][m].x), "+v"(acc[sl][m].y));
+      }
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (SPLIT == 2 && !GAIN) {
         // branch-free: spectra past the input weigh zero (finite values
@@ -395,59 +415,135 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     const long pitches = a.xg.rp | a.xg.cp;
     const bool vec = XL != XL_ROWS || (SPLIT == 2 ? ((pitches & 3) == 0) : ((pitches & 1) == 0));
     const int scnt_item = min(SPAN, a.Ns - s0);
-#pragma unroll
-    for (int m0 = 0; m0 < P; m0 += MR) {
-      __syncthreads();
-      tl = opaque(t);
-#pragma unroll
-      for (int mr = 0; mr < MR; mr++)
-#pragma unroll
-        for (int sl = 0; sl < SB; sl++)
-          buf[(mr * TPB + tl) * ROWW + SB * q + sl] =
-              (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
-      __syncthreads();
+    // Chunk c = tid + v*THREADS of a round is row c / CPR of the staging
+    // buffer, channel k = row % TPB + (m0 + row / TPB)*TPB, spectra
+    // s1 = (c % CPR)*SPC onwards.  Its X-input offset
+    // xg.chan(k) + xg.at(e, s0 + s1) is, in the layouts' own terms,
+    //   rows:   k*cp + e*rp + s0 + s1
+    //   pairs:  (blk*(N/2) + (k >> 1))*cp + (2e + (k & 1))*B + in_blk
+    //   blocks: (blk*N + k)*cp + e*B + in_blk
+    // with blk, in_blk the block of s0 + s1 and the place in it (s0, a
+    // multiple of SPAN, and e are uniform).
+    constexpr int LGB = (SPLIT == 2 ? 4 : 3) + (XL == XL_BLOCKS ? 1 : 0), B = 1 << LGB;
+    if constexpr (FeTuning<SPLIT>::lanes) {
+      // = ibase + m0*rstride + v*vstride + xoff, xoff the lane's part,
+      // computed once per item: with RV = THREADS / CPR rows per v, row =
+      // row0 + v*RV and k = row + m0*TPB; in the pairs layout k >> 1 =
+      // (row0 >> 1) + v*RV/2 + m0*TPB/2 and k & 1 = row0 & 1 (RV, TPB even)
+      constexpr int NV = (MR * TPB * CPR) / C::THREADS, RV = C::THREADS / CPR;
+      static_assert(C::THREADS % CPR == 0 && RV % 2 == 0 && TPB % 2 == 0, "chunk geometry");
+      const bool full = vec && scnt_item == SPAN;  // whole chunks only (uniform)
       const int tid2 = opaque(tid);
+      const int s1 = (tid2 % CPR) * SPC;  // the same for every v
+      const int row0 = tid2 / CPR;
+      const float2 *src0 = buf + row0 * ROWW + s1;
+      long xoff, ibase;
+      if constexpr (XL == XL_ROWS) {
+        xoff = (long)row0 * a.xg.cp + s1;
+        ibase = (long)e * a.xg.rp + s0;
+      } else if constexpr (XL == XL_PAIRS) {
+        xoff = (long)(row0 >> 1) * a.xg.cp + (row0 & 1) * B +
+               (SPAN >= B ? (long)(s1 >> LGB) * (N / 2) * a.xg.cp + (s1 & (B - 1)) : (long)s1);
+        ibase = (long)(s0 >> LGB) * (N / 2) * a.xg.cp + e * 2 * B + (SPAN >= B ? 0 : (s0 & (B - 1)));
+      } else {
+        xoff = (long)row0 * a.xg.cp + (SPAN >= B ? (long)(s1 >> LGB) * N * a.xg.cp + (s1 & (B - 1)) : (long)s1);
+        ibase = (long)(s0 >> LGB) * N * a.xg.cp + e * B + (SPAN >= B ? 0 : (s0 & (B - 1)));
+      }
+      const long rstride = (XL == XL_PAIRS ? (long)(TPB / 2) : (long)TPB) * a.xg.cp;  // per channel block of TPB
+      const long vstride = (XL == XL_PAIRS ? (long)(RV / 2) : (long)RV) * a.xg.cp;    // per RV rows
 #pragma unroll
-      for (int v = 0; v < (MR * TPB * CPR) / C::THREADS; v++) {
-        const int c = tid2 + v * C::THREADS;
-        const int row = c / CPR, cc = c % CPR;
-        const int k = row % TPB + (m0 + row / TPB) * TPB;
-        const int s1 = cc * SPC;  // first spectrum of the chunk, relative to s0
-        long o;  // xg.chan(k) + xg.at(e, s0 + s1)
-        if constexpr (XL == XL_ROWS) {
-          o = (long)k * a.xg.cp + (long)e * a.xg.rp + s0 + s1;
-        } else {
-          // s0 (uniform) is a multiple of SPAN, s1 < SPAN: keep the block
-          // index uniform where a block holds whole items
-          constexpr int LGB = (SPLIT == 2 ? 4 : 3) + (XL == XL_BLOCKS ? 1 : 0), B = 1 << LGB;
-          const int blk = SPAN >= B ? (s0 >> LGB) + (s1 >> LGB) : (s0 >> LGB);
-          const int in_blk = SPAN >= B ? (s1 & (B - 1)) : (s0 & (B - 1)) + s1;
-          if constexpr (XL == XL_PAIRS)
-            o = ((long)blk * (N / 2) + (k >> 1)) * a.xg.cp + ((e * 2 + (k & 1)) * B + in_blk);
-          else
-            o = ((long)blk * N + k) * a.xg.cp + (e * B + in_blk);
-        }
-        const float2 *src_ = buf + row * ROWW + s1;
-        if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
-          unsigned *d4 = (unsigned *)a.out + o;
-          const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
-          const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
-          if (vec && s1 + 3 < scnt_item) {
-            *(uint4 *)d4 = make_uint4(h0, h1, h2, h3);
-          } else {
-            if (s1 < scnt_item) d4[0] = h0;
-            if (s1 + 1 < scnt_item) d4[1] = h1;
-            if (s1 + 2 < scnt_item) d4[2] = h2;
-            if (s1 + 3 < scnt_item) d4[3] = h3;
+      for (int m0 = 0; m0 < P; m0 += MR) {
+        __syncthreads();
+        tl = opaque(t);
+#pragma unroll
+        for (int mr = 0; mr < MR; mr++)
+#pragma unroll
+          for (int sl = 0; sl < SB; sl++)
+            buf[(mr * TPB + tl) * ROWW + SB * q + sl] =
+                (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
+        __syncthreads();
+        const long rbase = ibase + (long)m0 * rstride + xoff;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const long o = rbase + v * vstride;
+          const float2 *src_ = src0 + v * RV * ROWW;
+          if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
+            unsigned *d4 = (unsigned *)a.out + o;
+            const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
+            const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
+            if (full || (vec && s1 + 3 < scnt_item)) {
+              *(uint4 *)d4 = make_uint4(h0, h1, h2, h3);
+            } else {
+              if (s1 < scnt_item) d4[0] = h0;
+              if (s1 + 1 < scnt_item) d4[1] = h1;
+              if (s1 + 2 < scnt_item) d4[2] = h2;
+              if (s1 + 3 < scnt_item) d4[3] = h3;
+            }
+          } else {  // 8-byte samples, 2 per chunk
+            float2 *dst = a.out + o;
+            const float2 v0 = src_[0], v1 = src_[1];
+            if (full || (vec && s1 + 1 < scnt_item)) {
+              *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+            } else {
+              if (s1 < scnt_item) dst[0] = v0;
+              if (s1 + 1 < scnt_item) dst[1] = v1;
+            }
           }
-        } else {  // 8-byte samples, 2 per chunk
-          float2 *dst = a.out + o;
-          const float2 v0 = src_[0], v1 = src_[1];
-          if (vec && s1 + 1 < scnt_item) {
-            *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+        }
+      }
+    } else {  // the offsets recomputed in every round
+#pragma unroll
+      for (int m0 = 0; m0 < P; m0 += MR) {
+        __syncthreads();
+        tl = opaque(t);
+#pragma unroll
+        for (int mr = 0; mr < MR; mr++)
+#pragma unroll
+          for (int sl = 0; sl < SB; sl++)
+            buf[(mr * TPB + tl) * ROWW + SB * q + sl] =
+                (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
+        __syncthreads();
+        const int tid2 = opaque(tid);
+#pragma unroll
+        for (int v = 0; v < (MR * TPB * CPR) / C::THREADS; v++) {
+          const int c = tid2 + v * C::THREADS;
+          const int row = c / CPR, cc = c % CPR;
+          const int k = row % TPB + (m0 + row / TPB) * TPB;
+          const int s1 = cc * SPC;  // first spectrum of the chunk, relative to s0
+          long o;
+          if constexpr (XL == XL_ROWS) {
+            o = (long)k * a.xg.cp + (long)e * a.xg.rp + s0 + s1;
           } else {
-            if (s1 < scnt_item) dst[0] = v0;
-            if (s1 + 1 < scnt_item) dst[1] = v1;
+            // keep the block index uniform where a block holds whole items
+            const int blk = SPAN >= B ? (s0 >> LGB) + (s1 >> LGB) : (s0 >> LGB);
+            const int in_blk = SPAN >= B ? (s1 & (B - 1)) : (s0 & (B - 1)) + s1;
+            if constexpr (XL == XL_PAIRS)
+              o = ((long)blk * (N / 2) + (k >> 1)) * a.xg.cp + ((e * 2 + (k & 1)) * B + in_blk);
+            else
+              o = ((long)blk * N + k) * a.xg.cp + (e * B + in_blk);
+          }
+          const float2 *src_ = buf + row * ROWW + s1;
+          if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
+            unsigned *d4 = (unsigned *)a.out + o;
+            const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
+            const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
+            if (vec && s1 + 3 < scnt_item) {
+              *(uint4 *)d4 = make_uint4(h0, h1, h2, h3);
+            } else {
+              if (s1 < scnt_item) d4[0] = h0;
+              if (s1 + 1 < scnt_item) d4[1] = h1;
+              if (s1 + 2 < scnt_item) d4[2] = h2;
+              if (s1 + 3 < scnt_item) d4[3] = h3;
+            }
+          } else {  // 8-byte samples, 2 per chunk
+            float2 *dst = a.out + o;
+            const float2 v0 = src_[0], v1 = src_[1];
+            if (vec && s1 + 1 < scnt_item) {
+              *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+            } else {
+              if (s1 < scnt_item) dst[0] = v0;
+              if (s1 + 1 < scnt_item) dst[1] = v1;
+            }
           }
         }
       }
