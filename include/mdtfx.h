@@ -211,6 +211,8 @@ typedef struct mdt_fx_plan_s *mdt_fx_plan;
 int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, int Ntaps,
                        int Nspectra, int window_kind, int xprec);
 int mdt_fx_set_stream(mdt_fx_plan plan, hipStream_t stream);
+/* The X input's sample size, and under MDT_XIN_AUTO its layout, follow the
+ * precision: query mdt_fx_xinput_bytes again after changing it. */
 int mdt_fx_set_precision(mdt_fx_plan plan, int xprec);
 /* bytes of the X input buffer the caller must provide (layout and precision
  * as set when it is filled) */
