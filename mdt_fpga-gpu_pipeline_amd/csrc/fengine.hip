@@ -191,17 +191,32 @@ struct PCfg {
   static constexpr int Q = QMAX < 1 ? 1 : (QMAX < 8 ? QMAX : 8);
   static constexpr int THREADS = Q * TPB;
   static constexpr int SPAN = SB * Q;                 // spectra per item
-  static constexpr int ROWW = SPAN + 1;               // staging row (float2), padded against bank conflicts
-  static constexpr int FFT_LDS = lds_entries<N>();    // per sub-block
   static constexpr int SPC = SPLIT == 2 ? 4 : 2;      // samples per 16-byte chunk of the output
   static constexpr int CPR = SPAN / SPC;              // chunks per row
+  // Staging rows (float2).  Rows of 8 spectra of 8 bytes (N = 4096): no pad,
+  // each 16-byte chunk stored at chunk ^ ((row >> 1) & 3), so that both the
+  // writers' and the readers' 16-byte accesses are free of bank conflicts;
+  // otherwise one float2 of pad per row.
+  static constexpr bool SWZ = SPAN == 8 && SPC == 2;
+  static constexpr int ROWW = SWZ ? SPAN : SPAN + 1;
+  static constexpr int FFT_LDS = lds_entries<N>();    // per sub-block
   // channel rounds per staging pass: as many as fit in the exchange buffers
-  static constexpr int MR0 = (Q * FFT_LDS) / (TPB * ROWW);
+  // (counted with padded rows)
+  static constexpr int MR0 = (Q * FFT_LDS) / (TPB * (SPAN + 1));
   static constexpr int MR = MR0 >= 4 ? 4 : (MR0 >= 2 ? 2 : 1);
   static constexpr int STAGE = MR * TPB * ROWW;
   static constexpr int BUF = Q * FFT_LDS > STAGE ? Q * FFT_LDS : STAGE;
   static constexpr int TWN = pass_tw_entries<N, P>();
 };
+
+// float2 index of staging column col (spectrum within the item) of row row
+template <class C>
+__device__ __forceinline__ int stage_idx(int row, int col) {
+  if constexpr (C::SWZ)
+    return row * C::ROWW + ((((col >> 1) ^ (row >> 1)) & 3) << 1) + (col & 1);
+  else
+    return row * C::ROWW + col;
+}
 
 __global__ void arrange_taps_kernel(int N, int T, const float *__restrict__ W, float *__restrict__ Wt) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,12 +455,12 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       // row0 + v*RV and k = row + m0*TPB; in the pairs layout k >> 1 =
       // (row0 >> 1) + v*RV/2 + m0*TPB/2 and k & 1 = row0 & 1 (RV, TPB even)
       constexpr int NV = (MR * TPB * CPR) / C::THREADS, RV = C::THREADS / CPR;
-      static_assert(C::THREADS % CPR == 0 && RV % 2 == 0 && TPB % 2 == 0, "chunk geometry");
+      static_assert(C::THREADS % CPR == 0 && RV % 8 == 0 && TPB % 2 == 0, "chunk geometry");
       const bool full = vec && scnt_item == SPAN;  // whole chunks only (uniform)
       const int tid2 = opaque(tid);
       const int s1 = (tid2 % CPR) * SPC;  // the same for every v
       const int row0 = tid2 / CPR;
-      const float2 *src0 = buf + row0 * ROWW + s1;
+      const float2 *src0 = buf + stage_idx<C>(row0, s1);  // chunk (s1, s1 + 1): rows row0 + v*RV share its swizzle
       long xoff, ibase;
       if constexpr (XL == XL_ROWS) {
         xoff = (long)row0 * a.xg.cp + s1;
@@ -468,7 +483,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         for (int mr = 0; mr < MR; mr++)
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
-            buf[(mr * TPB + tl) * ROWW + SB * q + sl] =
+            buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
                 (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
         __syncthreads();
         const long rbase = ibase + (long)m0 * rstride + xoff;
@@ -509,7 +524,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         for (int mr = 0; mr < MR; mr++)
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
-            buf[(mr * TPB + tl) * ROWW + SB * q + sl] =
+            buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
                 (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
         __syncthreads();
         const int tid2 = opaque(tid);
@@ -531,7 +546,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
             else
               o = ((long)blk * N + k) * a.xg.cp + (e * B + in_blk);
           }
-          const float2 *src_ = buf + row * ROWW + s1;
+          const float2 *src_ = buf + stage_idx<C>(row, s1);
           if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
             unsigned *d4 = (unsigned *)a.out + o;
             const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
