@@ -434,8 +434,9 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     // (channel) r of a round holds the item's SPAN spectra; each 16-byte
     // chunk of a row goes to one lane, the CPR chunks of a row to consecutive
     // lanes, so each store instruction writes whole SPAN-sample row pieces
-    // whole 16-byte chunks: always in the pairs layout (cp = NN*2B, B and the
-    // chunk offsets multiples of 4 / 2 samples), rows when the pitches allow
+    // whole 16-byte chunks: always in the pairs and blocks layouts (cp and B
+    // and the chunk offsets are multiples of the 4 / 2 samples of a chunk),
+    // in the rows layout when the pitches allow
     const long pitches = a.xg.rp | a.xg.cp;
     const bool vec = XL != XL_ROWS || (SPLIT == 2 ? ((pitches & 3) == 0) : ((pitches & 1) == 0));
     const int scnt_item = min(SPAN, a.Ns - s0);
@@ -460,7 +461,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       const int tid2 = opaque(tid);
       const int s1 = (tid2 % CPR) * SPC;  // the same for every v
       const int row0 = tid2 / CPR;
-      const float2 *src0 = buf + stage_idx<C>(row0, s1);  // chunk (s1, s1 + 1): rows row0 + v*RV share its swizzle
+      const float2 *src0 = buf + stage_idx<C>(row0, s1);  // the lane's chunk; rows row0 + v*RV share its swizzle
       long xoff, ibase;
       if constexpr (XL == XL_ROWS) {
         xoff = (long)row0 * a.xg.cp + s1;
