@@ -221,12 +221,9 @@ static_assert(WC_NB * 32 * WC_SEGS % WC_THREADS == 0, "chunk loads must divide a
 // pair, one pair per SIMD.  Waves 0-3 ("A", one per SIMD) take the five
 // blocks (r, 0..4) of row r = 7 - wave: one I row, J rows known at compile
 // time, straight-line code with no per-block branch.  Waves 4-7 ("B") take
-// the remaining four blocks of their pair from kWcB, reloading I per block.
+// the remaining four blocks of their pair (WcBw below), each wave's blocks
+// as constants of its own code path.
 constexpr int WC_A = 5, WC_B = 4;
-__constant__ unsigned char kWcB[4][WC_B][2] = {{{7, 5}, {7, 6}, {7, 7}, {0, 0}},
-                                               {{6, 5}, {6, 6}, {1, 0}, {1, 1}},
-                                               {{5, 5}, {2, 0}, {2, 1}, {2, 2}},
-                                               {{3, 0}, {3, 1}, {3, 2}, {3, 3}}};
 
 struct WcPrefetch {
   float4 v[WC_LOADS];
@@ -296,21 +293,50 @@ __device__ __forceinline__ void wc_chunk_a(const char *bufp, int irow, int l32, 
   }
 }
 
-// one chunk of a B wave: blocks (ra[b], rb[b]), I reloaded per block
-__device__ __forceinline__ void wc_chunk_b(const char *bufp, const int (&ra)[WC_B], const int (&rb)[WC_B], int l32,
-                                           int kh, f32x16 (&accR)[WC_B], f32x16 (&accI)[WC_B]) {
-#pragma unroll
-  for (int ks = 0; ks < WC_KC / 8; ks++) {
-    const int koff = ks * 32 + kh * 16;
-#pragma unroll
-    for (int b = 0; b < WC_B; b++) {
-      const int irow = ra[b] * 32 + l32, jrow = rb[b] * 32 + l32;
-      const half8 Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
-      const half8 Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
+// The B waves' blocks (row, column) as constants, wave 4 + W: consecutive
+// blocks of one row share its I operand (loaded and rotated once), and a
+// diagonal block's J operand is its I.  Per K-step a B wave reads 8 operand
+// rows from LDS instead of 16 (X-engine 7.22-7.25 -> 7.01-7.03 ms at 256
+// inputs, at the power limit); the MFMAs and their order are those of one
+// block after another, as in the A waves.
+template <int W> struct WcBw;
+template <> struct WcBw<0> { static constexpr int r[WC_B] = {7, 7, 7, 0}, c[WC_B] = {5, 6, 7, 0}; };
+template <> struct WcBw<1> { static constexpr int r[WC_B] = {6, 6, 1, 1}, c[WC_B] = {5, 6, 0, 1}; };
+template <> struct WcBw<2> { static constexpr int r[WC_B] = {5, 2, 2, 2}, c[WC_B] = {5, 0, 1, 2}; };
+template <> struct WcBw<3> { static constexpr int r[WC_B] = {3, 3, 3, 3}, c[WC_B] = {0, 1, 2, 3}; };
+
+template <int W, int B>
+__device__ __forceinline__ void wc_b_step(const char *bufp, int l32, int koff, half8 &Ih, half8 &Il, half8 &Ph,
+                                          half8 &Pl, f32x16 (&accR)[WC_B], f32x16 (&accI)[WC_B]) {
+  if constexpr (B < WC_B) {
+    constexpr int R = WcBw<W>::r[B], Cc = WcBw<W>::c[B];
+    if constexpr (B == 0 || R != WcBw<W>::r[B > 0 ? B - 1 : 0]) {
+      const int irow = R * 32 + l32;
+      Ih = *(const half8 *)(bufp + irow * WC_ROWB + koff);
+      Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
+      Ph = rot_conj(Ih);
+      Pl = rot_conj(Il);
+    }
+    if constexpr (Cc == R) {
+      wc_block(Ih, Il, Ph, Pl, Ih, Il, accR[B], accI[B]);
+    } else {
+      const int jrow = Cc * 32 + l32;
       const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
       const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
-      wc_block(Ih, Il, rot_conj(Ih), rot_conj(Il), Jh, Jl, accR[b], accI[b]);
+      wc_block(Ih, Il, Ph, Pl, Jh, Jl, accR[B], accI[B]);
     }
+    wc_b_step<W, B + 1>(bufp, l32, koff, Ih, Il, Ph, Pl, accR, accI);
+  }
+}
+
+// one chunk of B wave 4 + W
+template <int W>
+__device__ __forceinline__ void wc_chunk_b(const char *bufp, int l32, int kh, f32x16 (&accR)[WC_B],
+                                           f32x16 (&accI)[WC_B]) {
+#pragma unroll
+  for (int ks = 0; ks < WC_KC / 8; ks++) {
+    half8 Ih, Il, Ph, Pl;
+    wc_b_step<W, 0>(bufp, l32, ks * 32 + kh * 16, Ih, Il, Ph, Pl, accR, accI);
   }
 }
 
@@ -353,6 +379,26 @@ __device__ __forceinline__ void wc_loop(char *lds, const float2 *base, long ld, 
   }
 }
 
+// B wave 4 + W: its four blocks over all chunks, then their epilogue
+template <int W>
+__device__ __forceinline__ void wc_run_b(char *lds, const float2 *base, const XArgs &g, int batch, float oscale,
+                                         int NN, int K, int l32, int kh) {
+  int ra[WC_B], rb[WC_B];
+  f32x16 accR[WC_B], accI[WC_B];
+#pragma unroll
+  for (int b = 0; b < WC_B; b++) {
+    ra[b] = WcBw<W>::r[b];
+    rb[b] = WcBw<W>::c[b];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      accR[b][q] = 0.f;
+      accI[b][q] = 0.f;
+    }
+  }
+  wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_b<W>(bufp, l32, kh, accR, accI); });
+  wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
+}
+
 __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
   __shared__ __attribute__((aligned(16))) char lds[2 * WC_BUF];
 
@@ -384,20 +430,12 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
     wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_a(bufp, irow, l32, kh, accR, accI); });
     wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
   } else {
-    int ra[WC_B], rb[WC_B];
-    f32x16 accR[WC_B], accI[WC_B];
-#pragma unroll
-    for (int b = 0; b < WC_B; b++) {
-      ra[b] = kWcB[wave - 4][b][0];
-      rb[b] = kWcB[wave - 4][b][1];
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        accR[b][q] = 0.f;
-        accI[b][q] = 0.f;
-      }
+    switch (wave) {
+      case 4: wc_run_b<0>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
+      case 5: wc_run_b<1>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
+      case 6: wc_run_b<2>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
+      default: wc_run_b<3>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
     }
-    wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_b(bufp, ra, rb, l32, kh, accR, accI); });
-    wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
   }
 }
 
