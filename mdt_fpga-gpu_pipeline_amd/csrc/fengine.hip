@@ -271,13 +271,29 @@ constexpr int fe_waves_per_simd() {
 //  byte1: the input's high byte taken from the top of the word by an
 //         arithmetic shift, which converts with one SDWA byte-1 conversion
 //         instead of a 16-bit shift and a conversion (FP16 hi/lo 8.52
-//         against 8.62 ms; FP32 10.0 against 9.8-9.9 ms, FP8 unchanged).
+//         against 8.62 ms; FP32 10.0 against 9.8-9.9 ms, FP8 unchanged);
+//  stream: the X input written with non-temporal (streaming) 16-byte stores:
+//         it is read back only after the whole launch, far more data than
+//         the caches hold (FP16 hi/lo 8.35 against 8.46 ms, FP8 12.14
+//         against 13.05 ms; in the FP32 output's per-round stores 10.46
+//         against 9.86 ms).  Taken together with lanes.
 template <int SPLIT>
 struct FeTuning {
   static constexpr bool pin = SPLIT == 1;
   static constexpr bool lanes = SPLIT != 0;
   static constexpr bool byte1 = SPLIT == 1;
+  static constexpr bool stream = SPLIT != 0;
 };
+
+// one 16-byte store of four 32-bit words, streaming or not
+template <bool STREAM>
+__device__ __forceinline__ void store16(void *dst, unsigned w0, unsigned w1, unsigned w2, unsigned w3) {
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  if constexpr (STREAM)
+    __builtin_nontemporal_store(u32x4{w0, w1, w2, w3}, (u32x4 *)dst);
+  else
+    *(u32x4 *)dst = u32x4{w0, w1, w2, w3};
+}
 
 // X input layout with its block and row pitch as constants: XL_ROWS, XL_PAIRS
 // (B = 64 bytes of samples, row pitch 2B, block pitch (N/2) cp) or XL_BLOCKS
@@ -497,7 +513,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
             const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
             const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
             if (full || (vec && s1 + 3 < scnt_item)) {
-              *(uint4 *)d4 = make_uint4(h0, h1, h2, h3);
+              store16<FeTuning<SPLIT>::stream>(d4, h0, h1, h2, h3);
             } else {
               if (s1 < scnt_item) d4[0] = h0;
               if (s1 + 1 < scnt_item) d4[1] = h1;
@@ -508,7 +524,8 @@ fengine_persistent_kernel(FArgs a, long nitems) {
             float2 *dst = a.out + o;
             const float2 v0 = src_[0], v1 = src_[1];
             if (full || (vec && s1 + 1 < scnt_item)) {
-              *(float4 *)dst = make_float4(v0.x, v0.y, v1.x, v1.y);
+              store16<FeTuning<SPLIT>::stream>(dst, __float_as_uint(v0.x), __float_as_uint(v0.y),
+                                               __float_as_uint(v1.x), __float_as_uint(v1.y));
             } else {
               if (s1 < scnt_item) dst[0] = v0;
               if (s1 + 1 < scnt_item) dst[1] = v1;
