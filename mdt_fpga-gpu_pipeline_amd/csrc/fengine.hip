@@ -276,7 +276,8 @@ constexpr int fe_waves_per_simd() {
 //         it is read back only after the whole launch, far more data than
 //         the caches hold (FP16 hi/lo 8.35 against 8.46 ms, FP8 12.14
 //         against 13.05 ms; in the FP32 output's per-round stores 10.46
-//         against 9.86 ms).  Taken together with lanes.
+//         against 9.86 ms, and with lanes 10.50 against 9.80 ms).  Taken
+//         together with lanes.
 template <int SPLIT>
 struct FeTuning {
   static constexpr bool pin = SPLIT == 1;
