@@ -154,6 +154,82 @@ def stream_leg(torch, mdtfx, x8, dumps):
             "stack_bytes": stack_bytes}
 
 
+def station_leg(np, torch, mdtfx, steps, warmup):
+    """The MDT station shape (40 inputs, 4-tap PFB, 2048 channels, 4096
+    spectra per dump) on the drop-in calls, issued in the reference's rank-2
+    order per dump (MDT_OneStation_client.cu:1170 history carry, then 1083
+    PpS_Batch on the float F input, 1105 C2C FFT in place, 1114
+    ReorderFOutput, 1128 cublasCgemmBatched over the full NN x NN with
+    alpha = 1/Ns), every call bracketed by HIP events on the stream it runs
+    on; beside it the native fused path on the same samples.  Per-call
+    rooflines use each call's minimum bytes (or the GEMM's flops)."""
+    NN2, N2, T2, NS2 = 40, 2048, 4, 4096
+    log(f"bench: drop-in station leg NN={NN2} N={N2} T={T2} Ns={NS2}")
+    x8 = mdtfx.synth_i8(NN2, N2, NS2 + T2 - 1, seed=SEED, sigma=16.0, rho=RHO, max_delay=4)
+    W = mdtfx.pfb_window(N2, T2, mdtfx.WINDOW_BH4)
+    stream = torch.cuda.current_stream().cuda_stream
+    fft = mdtfx.FftPlan(N2, NS2 * NN2)
+    fft.set_stream(stream)
+    xe = mdtfx.XEngine()
+    xe.set_stream(stream)
+    f_in = torch.view_as_complex(x8.float().contiguous())  # [Ns+T-1][NN][N] complex64
+    F = torch.empty((NS2, NN2, N2), dtype=torch.complex64, device="cuda")
+    XIn = torch.empty((N2, NS2, NN2), dtype=torch.complex64, device="cuda")
+    XOut = torch.empty((N2, NN2, NN2), dtype=torch.complex64, device="cuda")
+    Aptr = torch.tensor([XIn.data_ptr() + c * NN2 * NS2 * 8 for c in range(N2)], dtype=torch.int64, device="cuda")
+    Cptr = torch.tensor([XOut.data_ptr() + c * NN2 * NN2 * 8 for c in range(N2)], dtype=torch.int64, device="cuda")
+    pts = NS2 * NN2 * N2
+    calls = [  # name, call, algorithmic bytes, flops
+        ("history_shift", lambda: mdtfx.history_shift(f_in, NN2, N2, T2, NS2), 2 * (T2 - 1) * NN2 * N2 * 8, 0),
+        ("pps_batch", lambda: mdtfx.pps_batch(W, f_in, NN2, NS2, F),
+         (NS2 + T2 - 1) * NN2 * N2 * 8 + T2 * N2 * 4 + pts * 8, 0),
+        ("fft_c2c", lambda: fft.exec_c2c(F, F, mdtfx.FFT_FORWARD), 16 * pts, 0),
+        ("reorder_foutput", lambda: mdtfx.reorder_foutput(F, NN2, N2, NS2, XIn), 16 * pts, 0),
+        ("cgemm_batched", lambda: xe.cgemm_batched(mdtfx.OP_N, mdtfx.OP_C, NN2, NN2, NS2, 1.0 / NS2, Aptr, NN2,
+                                                   Aptr, NN2, 0.0, Cptr, NN2, N2),
+         pts * 8 + N2 * NN2 * NN2 * 8, 8.0 * N2 * NS2 * NN2 * NN2),
+    ]
+    for _ in range(warmup):
+        for c in calls:
+            c[1]()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(calls) + 1)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record()
+        for i, c in enumerate(calls):
+            c[1]()
+            ev[k][i + 1].record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    per_call = {}
+    for i, (name, _, nbytes, flops) in enumerate(calls):
+        ms = sum(e[i].elapsed_time(e[i + 1]) for e in ev) / steps
+        r = {"ms": ms, "algorithmic_bytes": nbytes, "GB/s": nbytes / (ms * 1e-3) / 1e9,
+             "frac_hbm": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if flops:
+            r.update({"flops": flops, "TFLOP/s": flops / (ms * 1e-3) / 1e12,
+                      "frac_fp32_mfma": flops / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS})
+        per_call[name] = r
+    native = {}
+    for label, xprec in (("fp32", mdtfx.XPREC_FP32), ("fp16_split", mdtfx.XPREC_FP16_SPLIT)):
+        plan = mdtfx.FxPlan(NN2, N2, T2, NS2, mdtfx.WINDOW_BH4, xprec)
+        plan.set_stream(stream)
+        xin = plan.alloc_xinput()
+        vis = plan.alloc_vis()
+        w_ms, f_ms, x_ms = time_path(torch, mdtfx, plan, x8, xin, vis, None, steps, warmup)
+        native[label] = {"value": NS2 * N2 / (w_ms * 1e-3) / 1e6, "ms_per_dump": w_ms, "fengine_ms": f_ms,
+                         "xengine_ms": x_ms}
+        del plan, xin, vis
+    out = {"workload": "MDT station: 40 inputs, 4-tap PFB (BH4), 2048-pt FFT, 4096 spectra/dump; the "
+                       "reference's four drop-in calls + history carry per dump on the float F input",
+           "value": NS2 * N2 / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_dump": wall,
+           "steps": steps, "calls": per_call, "native_fused": native}
+    del fft, xe, f_in, F, XIn, XOut, x8
+    torch.cuda.empty_cache()
+    return out
+
+
 def load_traffic():
     """HBM bytes per launch from the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
     runs, scaled by the copy-kernel calibration (profiles/traffic_fx_dumps.json)."""
@@ -238,6 +314,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 error legs")
     ap.add_argument("--no-fp8", action="store_true", help="skip the wide-band FP8 leg")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-inclusive streaming leg")
+    ap.add_argument("--no-station", action="store_true", help="skip the drop-in station leg")
+    ap.add_argument("--station-only", action="store_true",
+                    help="only run the drop-in station leg and print its JSON (for per-kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -260,6 +339,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
+        return
+    if args.station_only:
+        print(json.dumps(station_leg(np, torch, mdtfx, args.steps, args.warmup)), flush=True)
         return
     stream = torch.cuda.current_stream().cuda_stream
     log(f"bench: scaled array NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
@@ -347,6 +429,8 @@ def main():
     if not args.no_fp8:
         out["fp8_wideband"] = fp8_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), stream,
                                 4, not args.no_cpu)
+    if not args.no_station:
+        out["station_dropin"] = station_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2))
     print(json.dumps(out), flush=True)
 
 
