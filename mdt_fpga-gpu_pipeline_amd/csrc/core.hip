@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void reorder_kernel(int NN, int N, int Ns,
 #pragma unroll
   for (int k = 0; k < RT_C / 8; k++) {
     int c = c0 + wy + 8 * k, e = e0 + wx;
-    if (e < NN && c < N) X[((long)c * Ns + s) * NN + e] = tile[wx][wy + 8 * k];
+    if (e < NN && c < N) store_stream(X + ((long)c * Ns + s) * NN + e, tile[wx][wy + 8 * k]);
   }
 }
 

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_kernel(const float2 *i
   if (valid) {
     float2 *dst = out + row * N + t;
 #pragma unroll
-    for (int m = 0; m < C::P; m++) dst[m * C::TPB] = x[m];
+    for (int m = 0; m < C::P; m++) store_stream(dst + m * C::TPB, x[m]);
   }
 }
 

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(PPS_THREADS) void pps_ring_kernel(int N, const floa
       re = __builtin_fmaf(w[i], xb[i].x, re);
       im = __builtin_fmaf(w[i], xb[i].y, im);
     }
-    o[(long)(s - s0) * spec] = make_float2(re, im);
+    store_stream(o + (long)(s - s0) * spec, make_float2(re, im));
 #pragma unroll
     for (int i = 0; i < T - 1; i++) xb[i] = xb[i + 1];
   }
