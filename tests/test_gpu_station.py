@@ -17,6 +17,11 @@ pytestmark = pytest.mark.gpu
 NN, N, T, NS, D = 40, 2048, 4, 4096, 8
 
 
+def _host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
 def _tri_autonorm_err(v, v64):
     a, b = np.tril_indices(NN)
     diag = np.array([orc.tri_index(i, i) for i in range(NN)])
@@ -79,3 +84,53 @@ def test_station_shape_dropin_eight_dumps():
             msg += f", mean normalised cross power {rho:.4f}"
             assert abs(rho - 0.3) < 0.01, msg
         print(msg, flush=True)
+
+
+def test_deployed_shape_dropin_and_native():
+    """The operating point the reference ships in its configuration
+    (MDT_SystemParams.conf:49-66: 19 elements x 2 pols = 38 inputs, 1024
+    channels, 8-tap PFB, 1024 spectra per dump): two dumps through the drop-in
+    sequence with the history carry, and the native FP32 and FP16-split paths
+    on the same windows, every channel against the FP64 oracle chain."""
+    NN2, N2, T2, NS2, D2 = 38, 1024, 8, 1024, 2
+    print(f"deployed NN={NN2} N={N2} T={T2} Ns={NS2} dumps={D2}", flush=True)
+    x8 = mdtfx.synth_i8(NN2, N2, D2 * NS2 + T2 - 1, seed=1907, sigma=16.0, rho=0.3, max_delay=4)
+    W = mdtfx.pfb_window(N2, T2, mdtfx.WINDOW_BH4)
+    W64 = W.cpu().numpy().astype(np.float64)
+    stream = torch.cuda.current_stream().cuda_stream
+    fft = mdtfx.FftPlan(N2, NS2 * NN2)
+    fft.set_stream(stream)
+    xe = mdtfx.XEngine()
+    xe.set_stream(stream)
+    f_in = torch.view_as_complex(x8[:NS2 + T2 - 1].float().contiguous())
+    F = torch.empty((NS2, NN2, N2), dtype=torch.complex64, device="cuda")
+    XIn = torch.empty((N2, NS2, NN2), dtype=torch.complex64, device="cuda")
+    XOut = torch.empty((N2, NN2, NN2), dtype=torch.complex64, device="cuda")
+    Aptr = torch.tensor([XIn.data_ptr() + c * NN2 * NS2 * 8 for c in range(N2)], dtype=torch.int64, device="cuda")
+    Cptr = torch.tensor([XOut.data_ptr() + c * NN2 * NN2 * 8 for c in range(N2)], dtype=torch.int64, device="cuda")
+    a, b = np.tril_indices(NN2)
+    diag = np.array([orc.tri_index(i, i) for i in range(NN2)])
+    chans = np.arange(N2)
+
+    def err(v, v64):
+        auto = np.real(v64[:, diag])
+        return float(np.max(np.abs(v - v64) / np.sqrt(auto[:, a] * auto[:, b])))
+
+    plans = {p: mdtfx.FxPlan(NN2, N2, T2, NS2, mdtfx.WINDOW_BH4, p) for p in (mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT)}
+    for d in range(D2):
+        if d > 0:
+            mdtfx.history_shift(f_in, NN2, N2, T2, NS2)
+            f_in[T2 - 1:] = torch.view_as_complex(x8[T2 - 1 + d * NS2: T2 - 1 + (d + 1) * NS2].float())
+        mdtfx.pps_batch(W, f_in, NN2, NS2, F)
+        fft.exec_c2c(F, F, mdtfx.FFT_FORWARD)
+        mdtfx.reorder_foutput(F, NN2, N2, NS2, XIn)
+        xe.cgemm_batched(mdtfx.OP_N, mdtfx.OP_C, NN2, NN2, NS2, 1.0 / NS2, Aptr, NN2, Aptr, NN2, 0.0, Cptr, NN2, N2)
+        dump = x8[d * NS2: d * NS2 + NS2 + T2 - 1].contiguous()
+        torch.cuda.synchronize()
+        v64 = orc.fx_chain_f64(W64, dump.cpu().numpy(), NN2, N2, T2, NS2, chans)
+        e = {"drop-in": err(XOut.cpu().numpy()[:, b, a].astype(np.complex128), v64)}
+        for p, plan in plans.items():
+            vis = plan.dump(dump, plan.alloc_xinput(), plan.alloc_vis())
+            e["fp32" if p == mdtfx.XPREC_FP32 else "fp16-split"] = err(_host(vis).astype(np.complex128), v64)
+        print(f"dump {d}: max normalised error vs FP64 " + ", ".join(f"{k} {v:.3e}" for k, v in e.items()), flush=True)
+        assert max(e.values()) < 1e-5, e
