@@ -217,8 +217,170 @@ __global__ __launch_bounds__(256) void xeng_f32_kernel(XArgs g) {
   }
 }
 
+// Hermitian drop-in products of small arrays (NN <= 48 and even, the MDT
+// station's 40 inputs) with the reference's X-input layout, where one
+// sample's NN rows are contiguous (ld == NN): a chunk of 64 samples is one
+// contiguous span of 64·NN complex values, loaded as 16-byte pieces into
+// registers while the previous chunk is multiplied.  16 x 16 blocks on
+// v_mfma_f32_16x16x4f32 pad 40 inputs to 48 rather than 64.  One workgroup
+// per batch entry; the nb(nb+1)/2 lower-triangle blocks are dealt to the four
+// waves, rotated per workgroup so that the waves with two blocks fall on
+// different SIMDs.  Operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16], K = 4 being two complex samples (k = 2t + re/im); D
+// register r of lane l is D[4(l/16) + r][l%16].  Products and epilogue as in
+// xeng_f32_kernel (XO_FULL, Hermitian).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int XS = 16;                       // block edge
+constexpr int XS_MAXN = 48;                  // largest NN
+constexpr int XS_KC = 64;                    // samples per chunk
+constexpr int XS_RL = XS_MAXN + 1;           // LDS plane row length (floats)
+constexpr int XS_PLANE = 2 * XS_RL;          // floats per sample: re row + im row
+constexpr int XS_V4 = XS_KC * XS_MAXN / 2 / 256;  // 16-byte pieces per thread per chunk (6)
+
+// al16 (uniform): the operand base is 16-byte aligned, so a piece is one
+// 16-byte load; otherwise two 8-byte loads (table entries are not checked on
+// the host)
+__device__ __forceinline__ void xs_fetch(float4 (&v)[XS_V4], const float2 *base, int NN, int s0, int K,
+                                         bool al16) {
+  const long nval = (long)(K - s0 < XS_KC ? K - s0 : XS_KC) * NN / 2;  // valid pieces (NN even)
+  const float2 *src = base + (long)s0 * NN;
+#pragma unroll
+  for (int i = 0; i < XS_V4; i++) {
+    const int q = threadIdx.x + i * 256;
+    if (q < nval) {
+      if (al16) {
+        v[i] = ((const float4 *)src)[q];
+      } else {
+        const float2 x0 = src[2 * q], x1 = src[2 * q + 1];
+        v[i] = make_float4(x0.x, x0.y, x1.x, x1.y);
+      }
+    } else {
+      v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+
+// pieces beyond the chunk's valid samples were zeroed by the fetch, so every
+// sample row of the chunk is written (rows NN..47 stay zero from the start)
+__device__ __forceinline__ void xs_stage(const float4 (&v)[XS_V4], float *lds, int NN) {
+  const int npiece = XS_KC * NN / 2;
+#pragma unroll
+  for (int i = 0; i < XS_V4; i++) {
+    const int q = threadIdx.x + i * 256;
+    if (q < npiece) {
+      const int e = 2 * q, s = e / NN, r = e - s * NN;  // r, r + 1 in the same sample (NN even)
+      float *p = lds + s * XS_PLANE + r;
+      p[0] = v[i].x;
+      p[XS_RL] = v[i].y;
+      p[1] = v[i].z;
+      p[XS_RL + 1] = v[i].w;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
+  __shared__ float lds[XS_KC * XS_PLANE];
+
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+  const int batch = (int)logical;
+  const float2 *base = g.Btab ? g.Btab[batch] : g.Bbase + (long)batch * g.Bbatch;
+  const int NN = g.m, K = g.k;
+  const int nb = (NN + XS - 1) / XS;
+  const int nblk = nb * (nb + 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int first = (wave + (int)(w & 3)) & 3;  // blocks first, first + 4
+  int bI[2], bJ[2];
+  bool has[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int b = first + 4 * q;
+    has[q] = b < nblk;
+    int P = 0;
+    while ((P + 1) * (P + 2) / 2 <= b) P++;
+    bI[q] = b - P * (P + 1) / 2;  // Q <= P: I rows <-> b, J rows <-> a (a >= b)
+    bJ[q] = P;
+  }
+  // two accumulator sets per block (alternate sample pairs), summed at the
+  // end: four to eight independent MFMA chains per wave instead of two to four
+  f32x4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        accR[q][h][r] = 0.f;
+        accI[q][h][r] = 0.f;
+      }
+  // rows NN..47 of every sample are zero for the whole launch
+  for (int i = threadIdx.x; i < XS_KC * XS_PLANE; i += 256) lds[i] = 0.f;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int t = kq >> 1, comp = kq & 1;
+
+  const bool al16 = ((uintptr_t)base & 15) == 0;
+  float4 v[XS_V4];
+  xs_fetch(v, base, NN, 0, K, al16);
+  for (int s0 = 0; s0 < K; s0 += XS_KC) {
+    __syncthreads();
+    xs_stage(v, lds, NN);
+    __syncthreads();
+    if (s0 + XS_KC < K) xs_fetch(v, base, NN, s0 + XS_KC, K, al16);
+#pragma unroll 2
+    for (int s = 0; s < XS_KC; s += 4) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const float *p = lds + (s + 2 * h + t) * XS_PLANE;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+          if (!has[q]) continue;
+          const int ri = bI[q] * XS + l16, rj = bJ[q] * XS + l16;
+          const float vi = p[comp * XS_RL + ri];
+          float vit = p[(1 - comp) * XS_RL + ri];
+          vit = comp ? vit : -vit;  // (-im, re)
+          const float vj = p[comp * XS_RL + rj];
+          accR[q][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(vi, vj, accR[q][h], 0, 0, 0);
+          accI[q][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(vit, vj, accI[q][h], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+  float2 *C = g.Ctab ? g.Ctab[batch] : g.Cbase + (long)batch * g.Cbatch;
+  const float2 al = g.alpha, be = g.beta;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (!has[q]) continue;
+    const int jg = bJ[q] * XS + l16;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int ig = bI[q] * XS + 4 * kq + r;
+      const float dre = accR[q][0][r] + accR[q][1][r], dim = accI[q][0][r] + accI[q][1][r];
+      const int a = jg, b = ig;  // V[a][b] = D, a >= b; mirror V[b][a] = conj(D)
+      if (a >= NN || b >= NN || a < b) continue;
+      {
+        const float2 c0 = (be.x != 0.f || be.y != 0.f) ? C[a + (long)b * g.ldc] : make_float2(0.f, 0.f);
+        C[a + (long)b * g.ldc] = make_float2(al.x * dre - al.y * dim + be.x * c0.x - be.y * c0.y,
+                                             al.x * dim + al.y * dre + be.x * c0.y + be.y * c0.x);
+      }
+      if (a != b) {
+        const float2 c0 = (be.x != 0.f || be.y != 0.f) ? C[b + (long)a * g.ldc] : make_float2(0.f, 0.f);
+        C[b + (long)a * g.ldc] = make_float2(al.x * dre + al.y * dim + be.x * c0.x - be.y * c0.y,
+                                             -al.x * dim + al.y * dre + be.x * c0.y + be.y * c0.x);
+      }
+    }
+  }
+}
+
 int xeng_launch(const XArgs &g0, int layout, int out, hipStream_t st) {
   XArgs g = g0;
+  if (layout == XL_KM && out == XO_FULL && g.herm && g.m <= XS_MAXN && (g.m & 1) == 0 && g.ldI == g.m) {
+    MDT_TRY(check_grid(g.batch, 256, "X-engine"));
+    hipLaunchKernelGGL(xeng_f32_small_kernel, dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    return check_launch("xeng_f32_small_kernel");
+  }
   const int nbM = (g.m + XT - 1) / XT;
   const int nbN = ((out == XO_TRI ? g.m : g.n) + XT - 1) / XT;
   if (g.herm) {

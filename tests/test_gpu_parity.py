@@ -161,7 +161,8 @@ def _autonorm_err(V, V64):
     return float(np.max(np.abs(V - V64) / norm))
 
 
-@pytest.mark.parametrize("NN,Ns,N", [(3, 2, 32), (38, 64, 32), (40, 256, 32), (64, 33, 32), (100, 129, 32), (130, 40, 32)])
+@pytest.mark.parametrize("NN,Ns,N", [(3, 2, 32), (38, 64, 32), (40, 256, 32), (64, 33, 32), (100, 129, 32), (130, 40, 32),
+                                     (40, 100, 32), (48, 33, 32), (4, 70, 32)])  # partial 64-sample chunks, NN 48 / 4
 def test_herk_vs_fp64(NN, Ns, N):
     print(f"herk NN={NN} Ns={Ns} N={N}", flush=True)
     rng = _rng(NN + Ns)
@@ -192,6 +193,33 @@ def test_cgemm_batched_reference_call(NN, Ns, N):
     V64 = orc.xengine_full(x, NN, Ns, N)
     assert np.all(np.isfinite(V))
     assert _autonorm_err(V, V64) < 1e-5
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+def test_cgemm_batched_hermitian_small(offset):
+    """The Hermitian call (one pointer table for A and B) at a small even NN
+    with complex alpha and beta, a padded ldc, a partial last chunk of
+    samples, and (offset 1) table entries one element past 16-byte alignment."""
+    rng = _rng(13 + offset)
+    m, k, ldc, batch = 40, 100, 44, 3
+    A = (rng.standard_normal((batch, k * m + 1)) + 1j * rng.standard_normal((batch, k * m + 1))).astype(np.complex64)
+    C0 = (rng.standard_normal((batch, m * ldc)) + 1j * rng.standard_normal((batch, m * ldc))).astype(np.complex64)
+    al, be = 0.5 - 0.25j, -1.0 + 2.0j
+    Ad, Cd = _to_gpu(A), _to_gpu(C0)
+    Aptr = torch.tensor([Ad.data_ptr() + (i * (k * m + 1) + offset) * 8 for i in range(batch)], dtype=torch.int64,
+                        device="cuda")
+    Cptr = torch.tensor([Cd.data_ptr() + i * m * ldc * 8 for i in range(batch)], dtype=torch.int64, device="cuda")
+    xe = mdtfx.XEngine()
+    xe.cgemm_batched(mdtfx.OP_N, mdtfx.OP_C, m, m, k, al, Aptr, m, Aptr, m, be, Cptr, ldc, batch)
+    C = _host(Cd)
+    for i in range(batch):
+        Ai = np.ascontiguousarray(A[i][offset:offset + k * m])
+        ref = orc.cgemm_nc(m, m, k, al, Ai, m, Ai, m, be, C0[i], ldc)
+        got = C[i].reshape(m, ldc).T[:m]
+        exp = ref.reshape(m, ldc).T[:m]
+        scale = np.sqrt(k) * 2 + np.abs(C0[i]).max() * 3
+        assert np.max(np.abs(got - exp)) < 1e-5 * scale
+        assert np.array_equal(C[i].reshape(m, ldc).T[m:], C0[i].reshape(m, ldc).T[m:])  # padding untouched
 
 
 def test_cgemm_batched_general():
