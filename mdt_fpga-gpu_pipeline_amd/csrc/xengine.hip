@@ -260,6 +260,41 @@ __device__ __forceinline__ void xs_fetch(float4 (&v)[XS_V4], const float2 *base,
   }
 }
 
+// The native X-input layouts (element (row e, sample s) at
+// e*ld + xin_spec(lgB, bp, s); ld, bp and the block length even, so a pair
+// of samples (s even) is one aligned 16-byte piece inside one block): piece q
+// is samples 2(q%32), 2(q%32)+1 of row q/32; samples past K are zeroed (the
+// pair's second sample lies in the row's or the block's padding then)
+__device__ __forceinline__ void xs_fetch_mk(float4 (&v)[XS_V4], const float2 *base, long ld, int lgB, long bp,
+                                            int NN, int s0, int K) {
+#pragma unroll
+  for (int i = 0; i < XS_V4; i++) {
+    const int q = threadIdx.x + i * 256;
+    const int row = q >> 5, s = s0 + 2 * (q & 31);
+    if (row < NN && s < K) {
+      float4 x = *(const float4 *)(base + row * ld + xin_spec(lgB, bp, s));
+      if (s + 1 >= K) x.z = x.w = 0.f;
+      v[i] = x;
+    } else {
+      v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+__device__ __forceinline__ void xs_stage_mk(const float4 (&v)[XS_V4], float *lds, int NN) {
+#pragma unroll
+  for (int i = 0; i < XS_V4; i++) {
+    const int q = threadIdx.x + i * 256;
+    const int r = q >> 5, s = 2 * (q & 31);
+    if (r < NN) {
+      float *p = lds + s * XS_PLANE + r;
+      p[0] = v[i].x;
+      p[XS_RL] = v[i].y;
+      p[XS_PLANE] = v[i].z;
+      p[XS_PLANE + XS_RL] = v[i].w;
+    }
+  }
+}
+
 // pieces beyond the chunk's valid samples were zeroed by the fetch, so every
 // sample row of the chunk is written (rows NN..47 stay zero from the start)
 __device__ __forceinline__ void xs_stage(const float4 (&v)[XS_V4], float *lds, int NN) {
@@ -278,6 +313,7 @@ __device__ __forceinline__ void xs_stage(const float4 (&v)[XS_V4], float *lds, i
   }
 }
 
+template <int LAYOUT, int OUT>
 __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
   __shared__ float lds[XS_KC * XS_PLANE];
 
@@ -285,7 +321,11 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
   const long w = blockIdx.x;
   const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
   const int batch = (int)logical;
-  const float2 *base = g.Btab ? g.Btab[batch] : g.Bbase + (long)batch * g.Bbatch;
+  const float2 *base;
+  if constexpr (OUT == XO_TRI)
+    base = g.Abase + xin_batch(g, batch);
+  else
+    base = g.Btab ? g.Btab[batch] : g.Bbase + (long)batch * g.Bbatch;
   const int NN = g.m, K = g.k;
   const int nb = (NN + XS - 1) / XS;
   const int nblk = nb * (nb + 1) / 2;
@@ -300,8 +340,9 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
     has[q] = b < nblk;
     int P = 0;
     while ((P + 1) * (P + 2) / 2 <= b) P++;
-    bI[q] = b - P * (P + 1) / 2;  // Q <= P: I rows <-> b, J rows <-> a (a >= b)
-    bJ[q] = P;
+    const int Q = b - P * (P + 1) / 2;  // Q <= P
+    bI[q] = (OUT == XO_TRI) ? P : Q;    // TRI: I rows <-> a; FULL: J rows <-> a (a >= b)
+    bJ[q] = (OUT == XO_TRI) ? Q : P;
   }
   // two accumulator sets per block (alternate sample pairs), summed at the
   // end: four to eight independent MFMA chains per wave instead of two to four
@@ -322,12 +363,23 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
 
   const bool al16 = ((uintptr_t)base & 15) == 0;
   float4 v[XS_V4];
-  xs_fetch(v, base, NN, 0, K, al16);
+  if constexpr (LAYOUT == XL_KM)
+    xs_fetch(v, base, NN, 0, K, al16);
+  else
+    xs_fetch_mk(v, base, g.ldI, g.xlgB, g.xbp, NN, 0, K);
   for (int s0 = 0; s0 < K; s0 += XS_KC) {
     __syncthreads();
-    xs_stage(v, lds, NN);
+    if constexpr (LAYOUT == XL_KM)
+      xs_stage(v, lds, NN);
+    else
+      xs_stage_mk(v, lds, NN);
     __syncthreads();
-    if (s0 + XS_KC < K) xs_fetch(v, base, NN, s0 + XS_KC, K, al16);
+    if (s0 + XS_KC < K) {
+      if constexpr (LAYOUT == XL_KM)
+        xs_fetch(v, base, NN, s0 + XS_KC, K, al16);
+      else
+        xs_fetch_mk(v, base, g.ldI, g.xlgB, g.xbp, NN, s0 + XS_KC, K);
+    }
 #pragma unroll 2
     for (int s = 0; s < XS_KC; s += 4) {
 #pragma unroll
@@ -358,6 +410,22 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
     for (int r = 0; r < 4; r++) {
       const int ig = bI[q] * XS + 4 * kq + r;
       const float dre = accR[q][0][r] + accR[q][1][r], dim = accI[q][0][r] + accI[q][1][r];
+      if constexpr (OUT == XO_TRI) {
+        const int a = ig, b = jg;  // V[a][b] = conj(D)
+        if (a < NN && b < NN && a >= b) {
+          const long o = (long)batch * g.Cbatch + tri_bl(a, b);
+          const float vr = al.x * dre, vim = -al.x * dim;
+          g.Cbase[o] = make_float2(vr, vim);
+          if (g.acc64) {
+            double2 *acc = (double2 *)g.acc64;
+            double2 tt = acc[o];
+            tt.x += (double)vr;
+            tt.y += (double)vim;
+            acc[o] = tt;
+          }
+        }
+        continue;
+      }
       const int a = jg, b = ig;  // V[a][b] = D, a >= b; mirror V[b][a] = conj(D)
       if (a >= NN || b >= NN || a < b) continue;
       {
@@ -378,7 +446,14 @@ int xeng_launch(const XArgs &g0, int layout, int out, hipStream_t st) {
   XArgs g = g0;
   if (layout == XL_KM && out == XO_FULL && g.herm && g.m <= XS_MAXN && (g.m & 1) == 0 && g.ldI == g.m) {
     MDT_TRY(check_grid(g.batch, 256, "X-engine"));
-    hipLaunchKernelGGL(xeng_f32_small_kernel, dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((xeng_f32_small_kernel<XL_KM, XO_FULL>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    return check_launch("xeng_f32_small_kernel");
+  }
+  // the native triangle, every X-input layout (even pitches: 16-byte aligned pieces)
+  if (layout == XL_MK && out == XO_TRI && g.herm && g.m <= XS_MAXN && (g.ldI & 1) == 0 && (g.xbp & 1) == 0 &&
+      g.xlgB >= 1) {
+    MDT_TRY(check_grid(g.batch, 256, "X-engine"));
+    hipLaunchKernelGGL((xeng_f32_small_kernel<XL_MK, XO_TRI>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
     return check_launch("xeng_f32_small_kernel");
   }
   const int nbM = (g.m + XT - 1) / XT;
