@@ -9,7 +9,11 @@ already resident in HBM.  The same dumps are timed with the FP32-MFMA
 X-engine for the metric's comparison.
 
 Prints ONE JSON line.  value = Msamples/s/input of the FP16-split path =
-Nspectra * Nchannels / (seconds per dump) / 1e6.
+Nspectra * Nchannels / (seconds per dump) / 1e6.  Beside it, each skippable:
+the host-inclusive streaming rate (--no-stream), the CPU port and the FP64
+error (--no-cpu), the 512-input FP8 wide-band shape (--no-fp8) and the
+reference's drop-in call sequence at the MDT station's shape (--no-station;
+--station-only runs that leg alone).
 """
 import argparse
 import json
