@@ -217,18 +217,22 @@ __global__ __launch_bounds__(256) void xeng_f32_kernel(XArgs g) {
   }
 }
 
-// Hermitian drop-in products of small arrays (NN <= 48 and even, the MDT
-// station's 40 inputs) with the reference's X-input layout, where one
-// sample's NN rows are contiguous (ld == NN): a chunk of 64 samples is one
-// contiguous span of 64·NN complex values, loaded as 16-byte pieces into
-// registers while the previous chunk is multiplied.  16 x 16 blocks on
-// v_mfma_f32_16x16x4f32 pad 40 inputs to 48 rather than 64.  One workgroup
-// per batch entry; the nb(nb+1)/2 lower-triangle blocks are dealt to the four
-// waves, rotated per workgroup so that the waves with two blocks fall on
-// different SIMDs.  Operand layout: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16], K = 4 being two complex samples (k = 2t + re/im); D
-// register r of lane l is D[4(l/16) + r][l%16].  Products and epilogue as in
-// xeng_f32_kernel (XO_FULL, Hermitian).
+// Hermitian products of small arrays (NN <= 48, the MDT station's 40
+// inputs): the drop-in CGEMM with the reference's X-input layout (NN even,
+// one sample's NN rows contiguous, ld == NN: a chunk of 64 samples is one
+// contiguous span of 64·NN complex values) and the native triangle in any
+// X-input layout (a chunk is 64 samples of each row), both loaded as 16-byte
+// pieces into registers while the previous chunk is multiplied.  16 x 16
+// blocks on v_mfma_f32_16x16x4f32 pad 40 inputs to 48 rather than 64, and
+// two accumulator sets per block (alternate sample pairs) give each wave four
+// to eight independent MFMA chains: with one set the dependent MFMAs'
+// latency set the time (2.30 against 1.21 ms at the station's shape).  One
+// workgroup per batch entry; the nb(nb+1)/2 lower-triangle blocks are dealt
+// to the four waves, rotated per workgroup so that the waves with two blocks
+// fall on different SIMDs.  Operand layout: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16], K = 4 being two complex samples (k = 2t + re/im); D
+// register r of lane l is D[4(l/16) + r][l%16].  Products and epilogues as
+// in xeng_f32_kernel.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int XS = 16;                       // block edge
 constexpr int XS_MAXN = 48;                  // largest NN
