@@ -140,7 +140,7 @@ def test_fft_tone_single_bin():
 
 
 # ----------------------------------------------------------------- a4 corner turn
-@pytest.mark.parametrize("NN,N,Ns", [(3, 32, 2), (38, 1024, 7), (40, 2048, 3), (70, 100, 5)])
+@pytest.mark.parametrize("NN,N,Ns", [(3, 32, 2), (38, 1024, 7), (40, 2048, 3), (70, 100, 5), (40, 100, 4), (64, 96, 3)])
 def test_reorder_bit_exact(NN, N, Ns):
     print(f"reorder NN={NN} N={N} Ns={Ns}", flush=True)
     rng = _rng(NN * N)
