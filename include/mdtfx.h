@@ -259,9 +259,13 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
 int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
 /* Per-input complex gain g_e and delay tau_e (samples) applied by the F-engine
  * to each channel k: X'_e[k] = g_e exp(-2 pi i k_s tau_e / N) X_e[k], k_s the
- * signed bin.  Device arrays of NN entries, read at every F-engine launch;
- * gains = NULL switches the stage off.  No counterpart in the reference
- * (calibration extension). */
+ * signed bin.  Device arrays of NN entries, read at every F-engine launch
+ * (they may be updated in place between launches); gains = NULL switches the
+ * stage off.  The FP16 / FP8 sample formats' power-of-two scale must then
+ * cover max|g| as well: each F-engine launch with gains derives it on the
+ * device, in stream order, and the X-engine divides by the scale the last
+ * F-engine launch used.  No host synchronisation.  No counterpart in the
+ * reference (calibration extension). */
 int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *delays);
 int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_complex *Vis,
                    double *Acc64);
@@ -270,7 +274,9 @@ int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNat
 /* FP8 plans: the per-channel exponents e_c the X-engine applies (device int
  * array of Nchannels): samples are quantised as e4m3(x*s*2^e_c), RMS ~ 32. */
 int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents);
-/* the plan's power-of-two scale s of the FP16 sample formats */
+/* the power-of-two scale s of the FP16 sample formats: the plan's, or with
+ * gains the one the last F-engine launch used (then this synchronises the
+ * plan's stream) */
 int mdt_fx_get_split_scale(mdt_fx_plan plan, float *scale);
 /* the plan's shape and precision (NULL outputs are skipped) */
 int mdt_fx_plan_shape(mdt_fx_plan plan, int *NpolsxNelements, int *Nchannels, int *Ntaps, int *Nspectra,
@@ -313,8 +319,17 @@ int mdt_stream_host_vis(mdt_stream stream, int slot, mdt_complex **vis);
  * zeros; the next dump starts a new stream of spectra.  No dump may be in
  * flight (the client's first block, client.cu:859-916). */
 int mdt_stream_prime(mdt_stream stream, const int8_t *history);
+/* Ordering with the caller's work.  The stream's three HIP streams are
+ * non-blocking; each submit first makes the dump wait for everything queued
+ * so far on the caller stream (the legacy default stream 0 unless set here),
+ * so buffers the caller prepared there -- a zeroed Acc64, gains -- are ready
+ * when the dump reads them. */
+int mdt_stream_set_caller_stream(mdt_stream stream, hipStream_t caller);
 /* Acc64: optional device FP64 accumulator [Nchannels][NN(NN+1)/2] complex,
- * V added in (as mdt_fx_xengine) */
+ * V added in (as mdt_fx_xengine).  Every dump in flight adds into it: the
+ * caller may read or write Acc64 only while no slot is in flight (every
+ * submitted slot waited for); zeroing it on the caller stream right before
+ * the first submit is ordered by the rule above. */
 int mdt_stream_submit(mdt_stream stream, int slot, double *Acc64);
 /* blocks until the slot's dump is done: its stack may be refilled and its
  * visibilities read; times (optional) gets the dump's phase times */

@@ -54,6 +54,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   const int scount = min(SB, a.Ns - s0);
   float2 *lds = lds_all + ey * C::SLICE;
   const int Tn = (T > 0) ? T : a.T;
+  const float sc = a.gain != nullptr ? *a.dscale : a.split_scale;
 
   float2 acc[SB][P];
 #pragma unroll
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
 #pragma unroll
       for (int m = 0; m < P; m++)
         a.out[a.xg.chan(t + m * TPB) + a.xg.at(e, s0)] =
-            a.split ? split_sample(acc[0][m], a.split_scale) : acc[0][m];
+            a.split ? split_sample(acc[0][m], sc) : acc[0][m];
     }
   } else {
     constexpr int CH = SB / 2;  // 16-byte chunks per (k, e) row segment
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
       __syncthreads();
 #pragma unroll
       for (int sl = 0; sl < SB; sl++)
-        lds[t * C::ROW + sl] = a.split ? split_sample(acc[sl][m], a.split_scale) : acc[sl][m];
+        lds[t * C::ROW + sl] = a.split ? split_sample(acc[sl][m], sc) : acc[sl][m];
       __syncthreads();
       if (valid_e) {
 #pragma unroll
@@ -312,6 +313,9 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   __shared__ float2 twl[C::TWN];
   __shared__ float2 buf[C::BUF];
   const int tid = threadIdx.x;
+  // the FP16 formats' scale: with gains it covers |g| x the bound (computed on
+  // the device before the launch)
+  const float split_sc = GAIN ? *a.dscale : a.split_scale;
   // sub-block: uniform within a wave (TPB >= 64); made a scalar explicitly so
   // that frame offsets and buffer bases stay in scalar registers
   const int q = __builtin_amdgcn_readfirstlane(tid / TPB);
@@ -415,7 +419,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       if constexpr (SPLIT == 2 && !GAIN) {
         // branch-free: spectra past the input weigh zero (finite values
         // times 0, so pw is unchanged bit for bit)
-        const float wgt = sl < scount ? a.split_scale : 0.f;
+        const float wgt = sl < scount ? split_sc : 0.f;
 #pragma unroll
         for (int m = 0; m < P; m++) {
           const float xr = acc[sl][m].x * wgt, xi = acc[sl][m].y * wgt;
@@ -440,7 +444,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         for (int m = 0; m < P; m++)
 #pragma unroll
           for (int sl = 0; sl < SB; sl++) {
-            const float wgt = sl < scount ? a.split_scale : 0.f;
+            const float wgt = sl < scount ? split_sc : 0.f;
             const float xr = acc[sl][m].x * wgt, xi = acc[sl][m].y * wgt;
             pw[m] = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, pw[m]));
           }
@@ -502,7 +506,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
             buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
-                (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
+                (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], split_sc) : acc[sl][m0 + mr];
         __syncthreads();
         const long rbase = ibase + (long)m0 * rstride + xoff;
 #pragma unroll
@@ -511,8 +515,8 @@ fengine_persistent_kernel(FArgs a, long nitems) {
           const float2 *src_ = src0 + v * RV * ROWW;
           if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
             unsigned *d4 = (unsigned *)a.out + o;
-            const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
-            const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
+            const unsigned h0 = half_sample(src_[0], split_sc), h1 = half_sample(src_[1], split_sc);
+            const unsigned h2 = half_sample(src_[2], split_sc), h3 = half_sample(src_[3], split_sc);
             if (full || (vec && s1 + 3 < scnt_item)) {
               store16<FeTuning<SPLIT>::stream>(d4, h0, h1, h2, h3);
             } else {
@@ -544,7 +548,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
             buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
-                (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], a.split_scale) : acc[sl][m0 + mr];
+                (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], split_sc) : acc[sl][m0 + mr];
         __syncthreads();
         const int tid2 = opaque(tid);
 #pragma unroll
@@ -568,8 +572,8 @@ fengine_persistent_kernel(FArgs a, long nitems) {
           const float2 *src_ = buf + stage_idx<C>(row, s1);
           if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
             unsigned *d4 = (unsigned *)a.out + o;
-            const unsigned h0 = half_sample(src_[0], a.split_scale), h1 = half_sample(src_[1], a.split_scale);
-            const unsigned h2 = half_sample(src_[2], a.split_scale), h3 = half_sample(src_[3], a.split_scale);
+            const unsigned h0 = half_sample(src_[0], split_sc), h1 = half_sample(src_[1], split_sc);
+            const unsigned h2 = half_sample(src_[2], split_sc), h3 = half_sample(src_[3], split_sc);
             if (vec && s1 + 3 < scnt_item) {
               *(uint4 *)d4 = make_uint4(h0, h1, h2, h3);
             } else {

@@ -17,9 +17,20 @@ struct FArgs {
   const float *delay;   // optional [NN] delays in samples
   int split;            // 0: FP32 (re, im); 1: FP16 (hi_re, hi_im, lo_re, lo_im) of x*split_scale;
                         // 2: FP16 (re, im) of x*split_scale in 4-byte elements + channel power
-  float split_scale;    // exact power of two
+  float split_scale;    // exact power of two (without gains)
+  const float *dscale;  // with gains: the power-of-two scale for |g| x the bound, on the device
   float *power;         // split 2: per-channel sum of |x*split_scale|^2 (zeroed by the caller)
 };
+
+// The FP16 sample formats store x*s, s the largest power of two with
+// s*bound <= 2^15 (below the FP16 maximum 65504), bound a rigorous bound of
+// |x|.  Exact: with bound = m 2^E, m in [0.5, 1), s = 2^(15-E), or 2^(16-E)
+// when m = 0.5.
+__host__ __device__ inline float f16_prescale(double bound) {
+  int E = 0;
+  const double m = frexp(bound, &E);
+  return ldexpf(1.0f, 15 - E + (m == 0.5 ? 1 : 0));
+}
 
 // FP16 (re, im) of a sample scaled by an exact power of two, as 4 bytes
 __device__ __forceinline__ unsigned half_sample(float2 v, float sc) {

@@ -20,7 +20,10 @@ struct mdt_fx_plan_s {
   float2 *tw = nullptr;
   const float2 *gain = nullptr;
   const float *delay = nullptr;
-  float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15
+  float f16_scale = 1.0f;  // exact power of two keeping |X| * scale <= 2^15 (no gains)
+  double wbound = 0.0;     // rigorous bound of |X| without gains
+  float *dscale = nullptr; // device: the scale of the last F-engine launch (f16_scale, or with gains)
+  bool dscale_base = true; // dscale holds f16_scale
   float *power = nullptr;  // FP8 path: per-channel power from the F-engine
   float *Wt = nullptr;     // window arranged for the persistent F-engine (when it serves the shape)
   int layout = MDT_XIN_AUTO;
@@ -81,18 +84,40 @@ using namespace mdt;
 
 // Rigorous bound of the channelised samples for int8 input: |X[k]| <=
 // sum_n |y[n]| <= |x|max * sum_{i,n} |W[i][n]|, |x|max = |(-128, -128)|.  The
-// FP16 split pre-scale is the power of two that maps this bound to <= 2^15,
-// below the FP16 maximum 65504; with gains applied the bound grows by max|g|.
-static int choose_f16_scale(mdt_fx_plan p, float gmax = 1.0f) {
+// FP16 pre-scale is the power of two that maps this bound to <= 2^15, below
+// the FP16 maximum 65504 (f16_prescale).  Computed once, at plan creation.
+static int choose_f16_scale(mdt_fx_plan p) {
   std::vector<float> w((size_t)p->T * p->N);
   if (hipMemcpy(w.data(), p->W, sizeof(float) * w.size(), hipMemcpyDeviceToHost) != hipSuccess)
     return fail(MDT_ERR_DEVICE, "window read-back");
   double sum = 0.0;
   for (float v : w) sum += std::fabs((double)v);
-  const double bound = 181.02 * sum * (double)gmax;
-  const int e = (int)std::floor(std::log2(32768.0 / bound));
-  p->f16_scale = std::ldexp(1.0f, e);
+  p->wbound = 181.02 * sum;
+  p->f16_scale = f16_prescale(p->wbound);
+  if (hipMemcpy(p->dscale, &p->f16_scale, sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(MDT_ERR_DEVICE, "scale upload");
   return MDT_SUCCESS;
+}
+
+// With gains the bound grows by max(max_e |g_e|, 1): the F-engine's scale is
+// computed on the device, in stream order before the F-engine reads it, and
+// stays there for the X-engine that divides by it.  gains = NULL writes the
+// plan's scale back.  One workgroup, fixed-order maximum.
+__global__ void gain_scale_kernel(const float2 *gains, int NN, double wbound, float *dscale) {
+  __shared__ double red[256];
+  double m = 0.0;
+  if (gains != nullptr)
+    for (int e = threadIdx.x; e < NN; e += 256) {
+      const float2 g = gains[e];
+      m = fmax(m, sqrt((double)g.x * g.x + (double)g.y * g.y));
+    }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *dscale = f16_prescale(wbound * fmax(red[0], 1.0));
 }
 
 __global__ void zero_kernel(float *p, int n) {
@@ -143,6 +168,12 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
     delete p;
     return fail(MDT_ERR_ALLOC, "channel power");
   }
+  if (hipMalloc(&p->dscale, sizeof(float)) != hipSuccess) {
+    (void)hipFree(p->W);
+    (void)hipFree(p->power);
+    delete p;
+    return fail(MDT_ERR_ALLOC, "scale");
+  }
   int st = launch_window(Nchannels, Ntaps, p->W, window_kind, 0);
   if (st == MDT_SUCCESS) {
     hipLaunchKernelGGL(zero_kernel, dim3((Nchannels + 255) / 256), dim3(256), 0, 0, p->power, Nchannels);
@@ -161,6 +192,7 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
   if (st != MDT_SUCCESS) {
     (void)hipFree(p->W);
     (void)hipFree(p->power);
+    (void)hipFree(p->dscale);
     if (p->Wt) (void)hipFree(p->Wt);
     if (p->tw) (void)hipFree(p->tw);
     delete p;
@@ -186,18 +218,9 @@ int mdt_fx_set_precision(mdt_fx_plan plan, int xprec) {
 int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *delays) {
   MDT_REQUIRE_PTR(plan);
   if (!gains && delays) return fail(MDT_ERR_NULL_POINTER, "delays need gains");
-  float gmax = 1.0f;
-  if (gains) {
-    std::vector<float2> h(plan->NN);
-    if (hipMemcpy(h.data(), gains, sizeof(float2) * plan->NN, hipMemcpyDeviceToHost) != hipSuccess)
-      return fail(MDT_ERR_DEVICE, "gain read-back");
-    gmax = 0.0f;
-    for (const float2 &v : h) gmax = std::fmax(gmax, std::sqrt(v.x * v.x + v.y * v.y));
-    gmax = std::fmax(gmax, 1.0f);
-  }
-  plan->gain = (const float2 *)gains;
+  plan->gain = (const float2 *)gains;  // read (and the scale derived) at each F-engine launch
   plan->delay = delays;
-  return choose_f16_scale(plan, gmax);  // the split scale must cover |g| x bound
+  return MDT_SUCCESS;
 }
 
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
@@ -281,7 +304,14 @@ int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInput
   a.delay = plan->delay;
   a.split = (plan->xprec == MDT_XPREC_FP16_SPLIT) ? 1 : (plan->xprec == MDT_XPREC_FP8 ? 2 : 0);
   a.split_scale = plan->f16_scale;
+  a.dscale = plan->dscale;
   a.power = plan->power;
+  if (a.split != 0 && (plan->gain != nullptr || !plan->dscale_base)) {
+    hipLaunchKernelGGL(gain_scale_kernel, dim3(1), dim3(256), 0, plan->stream, (const float2 *)plan->gain,
+                       plan->NN, plan->wbound, plan->dscale);
+    MDT_TRY(check_launch("gain_scale_kernel"));
+    plan->dscale_base = plan->gain == nullptr;
+  }
   if (a.split == 2) {  // graph-safe zeroing (a kernel, not a memset node)
     hipLaunchKernelGGL(zero_kernel, dim3((plan->N + 255) / 256), dim3(256), 0, plan->stream, plan->power,
                        plan->N);
@@ -315,6 +345,7 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
   g.acc64 = Acc64;
   g.prec = plan->xprec;
   g.in_scale = plan->f16_scale;
+  g.scale_dev = plan->dscale;  // as the last F-engine launch used it
   if (plan->xprec == MDT_XPREC_FP16_SPLIT) return xeng_launch_f16split(g, plan->stream);
   if (plan->xprec == MDT_XPREC_FP8) return xeng_launch_f8(g, plan->power, plan->stream);
   return xeng_launch(g, XL_MK, XO_TRI, plan->stream);
@@ -348,7 +379,12 @@ int mdt_fx_plan_shape(mdt_fx_plan plan, int *NpolsxNelements, int *Nchannels, in
 int mdt_fx_get_split_scale(mdt_fx_plan plan, float *scale) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(scale);
-  *scale = plan->f16_scale;
+  if (plan->dscale_base) {
+    *scale = plan->f16_scale;
+    return MDT_SUCCESS;
+  }
+  MDT_HIP_TRY(hipMemcpyAsync(scale, plan->dscale, sizeof(float), hipMemcpyDeviceToHost, plan->stream));
+  MDT_HIP_TRY(hipStreamSynchronize(plan->stream));
   return MDT_SUCCESS;
 }
 
@@ -356,6 +392,7 @@ int mdt_fx_plan_destroy(mdt_fx_plan plan) {
   if (!plan) return MDT_SUCCESS;
   if (plan->W) (void)hipFree(plan->W);
   if (plan->power) (void)hipFree(plan->power);
+  if (plan->dscale) (void)hipFree(plan->dscale);
   if (plan->Wt) (void)hipFree(plan->Wt);
   if (plan->tw) (void)hipFree(plan->tw);
   delete plan;

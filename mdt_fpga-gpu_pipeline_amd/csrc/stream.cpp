@@ -17,7 +17,11 @@
 // used twice in a row) the tail it reads is not yet overwritten: the tail
 // starts at spectrum Ns >= T-1, past the head it writes.  The one X input is
 // shared by all dumps: its writer (F) and reader (X) are on the compute
-// stream, which orders them.
+// stream, which orders them.  The three streams are non-blocking, so before
+// anything else a dump's copy-in and compute streams wait for an event
+// recorded on the caller's stream at submit: what the caller queued there
+// first (zeroing the FP64 accumulator, writing gains) is complete before the
+// dump reads it.
 #include <new>
 
 #include "common.hpp"
@@ -49,6 +53,8 @@ struct mdt_stream_s {
   size_t xin_bytes = 0;  // the X input allocated for the plan as it was at creation
   mdt_complex *vis[kSlots] = {};
   Ev ev[kSlots];
+  hipStream_t caller = 0;        // submits order after the work queued here
+  hipEvent_t caller_ev = nullptr;
   bool busy[kSlots] = {};
   int prev = -1;  // slot of the last submitted dump, -1: the prime buffer
 };
@@ -67,6 +73,7 @@ static void release(mdt_stream s) {
     for (hipEvent_t *p : e)
       if (*p) (void)hipEventDestroy(*p);
   }
+  if (s->caller_ev) (void)hipEventDestroy(s->caller_ev);
   if (s->prime) (void)hipFree(s->prime);
   if (s->xin) (void)hipFree(s->xin);
   if (s->cin) (void)hipStreamDestroy(s->cin);
@@ -111,6 +118,7 @@ int mdt_stream_create(mdt_stream *stream, mdt_fx_plan plan) {
     ok(hipStreamCreateWithFlags(&s->cin, hipStreamNonBlocking), "copy-in stream");
     ok(hipStreamCreateWithFlags(&s->comp, hipStreamNonBlocking), "compute stream");
     ok(hipStreamCreateWithFlags(&s->cout, hipStreamNonBlocking), "copy-out stream");
+    ok(hipEventCreateWithFlags(&s->caller_ev, hipEventDisableTiming), "caller event");
     ok(hipMalloc(&s->xin, xbytes), "X input");
     s->xin_bytes = xbytes;
     ok(hipMalloc(&s->prime, hist > 0 ? hist : 16), "history buffer");
@@ -175,6 +183,10 @@ static int enqueue(mdt_stream s, int slot, double *Acc64) {
   Ev &e = s->ev[slot];
   const size_t hist = s->spectrum * (s->T - 1), stack = s->spectrum * s->Ns;
   const int8_t *src = s->prev < 0 ? s->prime : s->in[s->prev] + stack;  // previous dump's last T-1 spectra
+  // the caller's earlier work (accumulator zeroing, gains) comes first
+  MDT_HIP_TRY(hipEventRecord(s->caller_ev, s->caller));
+  MDT_HIP_TRY(hipStreamWaitEvent(s->cin, s->caller_ev, 0));
+  MDT_HIP_TRY(hipStreamWaitEvent(s->comp, s->caller_ev, 0));
   // copy-in
   MDT_HIP_TRY(hipStreamWaitEvent(s->cin, e.in_free, 0));
   MDT_HIP_TRY(hipEventRecord(e.t0, s->cin));
@@ -185,6 +197,7 @@ static int enqueue(mdt_stream s, int slot, double *Acc64) {
   MDT_HIP_TRY(hipStreamWaitEvent(s->comp, e.h2d_done, 0));
   MDT_HIP_TRY(hipStreamWaitEvent(s->comp, e.vis_free, 0));
   MDT_HIP_TRY(hipEventRecord(e.f0, s->comp));
+  MDT_TRY(mdt_fx_set_stream(s->plan, s->comp));  // whatever the plan was last bound to
   MDT_TRY(mdt_fx_fengine(s->plan, s->in[slot], (mdt_complex *)s->xin));
   MDT_HIP_TRY(hipEventRecord(e.in_free, s->comp));
   MDT_TRY(mdt_fx_xengine(s->plan, (const mdt_complex *)s->xin, s->vis[slot], Acc64));
@@ -193,6 +206,12 @@ static int enqueue(mdt_stream s, int slot, double *Acc64) {
   MDT_HIP_TRY(hipStreamWaitEvent(s->cout, e.x_done, 0));
   MDT_HIP_TRY(hipMemcpyAsync(s->host_vis[slot], s->vis[slot], s->vis_bytes, hipMemcpyDeviceToHost, s->cout));
   MDT_HIP_TRY(hipEventRecord(e.vis_free, s->cout));
+  return MDT_SUCCESS;
+}
+
+int mdt_stream_set_caller_stream(mdt_stream stream, hipStream_t caller) {
+  MDT_REQUIRE_PTR(stream);
+  stream->caller = caller;
   return MDT_SUCCESS;
 }
 

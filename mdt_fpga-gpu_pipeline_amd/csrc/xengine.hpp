@@ -31,6 +31,7 @@ struct XArgs {
   double *acc64;  // XO_TRI only: FP64 complex accumulator, += V (nullable)
   int prec;       // MDT_XPREC_*
   float in_scale; // FP16 split: exact power-of-two pre-scale of the FP32 samples
+  const float *scale_dev = nullptr;  // the same scale on the device, as the F-engine used it (nullable)
   // native X input geometry (XGeom, common.hpp) beyond the row pitch: batch
   // b's operand starts at base + (b >> xlgG) * Abatch + (b mod 2^xlgG) *
   // 2^xlgB, sample (row, s) at row * ld + (s >> xlgB) * xbp + (s mod

@@ -88,6 +88,10 @@ __device__ __forceinline__ void stage(const Prefetch &p, char *lds) {
 
 
 __global__ __launch_bounds__(256, 2) void xeng_f16split_kernel(XArgs g, float oscale) {
+  if (g.scale_dev != nullptr) {  // the scale the F-engine used (gains change it)
+    const float sc = *g.scale_dev;
+    oscale = g.alpha.x / (sc * sc);
+  }
   __shared__ __attribute__((aligned(16))) char lds[2 * 2 * HPLANE];  // [op I|J][hi|lo][row][k]
 
   const long nwg = gridDim.x;
@@ -404,6 +408,10 @@ __device__ __forceinline__ void wc_run_b(char *lds, const float2 *base, const XA
 }
 
 __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
+  if (g.scale_dev != nullptr) {  // the scale the F-engine used (gains change it)
+    const float sc = *g.scale_dev;
+    oscale = g.alpha.x / (sc * sc);
+  }
   __shared__ __attribute__((aligned(16))) char lds[2 * WC_BUF];
 
   const long nwg = gridDim.x;

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *p
 
   const int ec = chan_exponent(power[batch], count);
   const float inv_scale = exp2f((float)-ec);  // the conversion divides by this
-  const float s_total = g.in_scale * exp2f((float)ec);
+  const float s_total = (g.scale_dev != nullptr ? *g.scale_dev : g.in_scale) * exp2f((float)ec);
   const float oscale = g.alpha.x / (s_total * s_total);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

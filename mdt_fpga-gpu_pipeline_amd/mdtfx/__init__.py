@@ -120,6 +120,7 @@ _SIGS = {
     "mdt_stream_host_stack": [_VP, _I, ctypes.POINTER(_VP)],
     "mdt_stream_host_vis": [_VP, _I, ctypes.POINTER(_VP)],
     "mdt_stream_prime": [_VP, _VP],
+    "mdt_stream_set_caller_stream": [_VP, _VP],
     "mdt_stream_submit": [_VP, _I, _VP],
     "mdt_stream_wait": [_VP, _I, _VP],
     "mdt_stream_destroy": [_VP],
@@ -237,13 +238,19 @@ class FftPlan:
         self._h = ctypes.c_void_p()
         check(lib().mdt_fft_plan_1d(ctypes.byref(self._h), nx, type, batch))
         self.nx, self.batch = nx, batch
+        self._bound = None
 
     def set_stream(self, stream_handle):
+        """bind the plan to a HIP stream; unbound plans run on torch's
+        current stream at each call"""
+        self._bound = stream_handle
         check(lib().mdt_fft_set_stream(self._h, ctypes.c_void_p(stream_handle)))
 
     def exec_c2c(self, idata, odata, direction=FFT_FORWARD):
         _need(idata, self.nx * self.batch, "idata")
         _need(odata, self.nx * self.batch, "odata")
+        if self._bound is None:
+            check(lib().mdt_fft_set_stream(self._h, _stream()))
         check(lib().mdt_fft_exec_c2c(self._h, _ptr(idata, torch.complex64),
                                      _ptr(odata, torch.complex64), direction))
         return odata
@@ -277,9 +284,17 @@ class XEngine:
     def __init__(self):
         self._h = ctypes.c_void_p()
         check(lib().mdt_xengine_create(ctypes.byref(self._h)))
+        self._bound = None
 
     def set_stream(self, stream_handle):
+        """bind the handle to a HIP stream; unbound handles run on torch's
+        current stream at each call"""
+        self._bound = stream_handle
         check(lib().mdt_xengine_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def _sync_stream(self):
+        if self._bound is None:
+            check(lib().mdt_xengine_set_stream(self._h, _stream()))
 
     def cgemm_batched(self, transa, transb, m, n, k, alpha, Aptrs, lda, Bptrs, ldb, beta,
                       Cptrs, ldc, batch):
@@ -288,6 +303,7 @@ class XEngine:
             _need(t, batch, name)
         al = _Complex(float(complex(alpha).real), float(complex(alpha).imag))
         be = _Complex(float(complex(beta).real), float(complex(beta).imag))
+        self._sync_stream()
         check(lib().mdt_cgemm_batched(self._h, transa, transb, m, n, k, ctypes.byref(al),
                                       _ptr(Aptrs, torch.int64), lda, _ptr(Bptrs, torch.int64), ldb,
                                       ctypes.byref(be), _ptr(Cptrs, torch.int64), ldc, batch))
@@ -295,6 +311,7 @@ class XEngine:
     def herk(self, NN, Ns, alpha, XIn, XOut, N):
         _need(XIn, N * Ns * NN, "XIn")
         _need(XOut, N * NN * NN, "XOut")
+        self._sync_stream()
         check(lib().mdt_xengine_herk(self._h, NN, Ns, float(alpha), _ptr(XIn, torch.complex64),
                                      _ptr(XOut, torch.complex64), N))
         return XOut
@@ -340,6 +357,12 @@ class FxPlan:
         self.NN, self.N, self.T, self.Ns = NN, N, T, Ns
         self.nb = NN * (NN + 1) // 2
         self.xprec = xprec
+        self._bound = None
+
+    def _sync_stream(self):
+        """unbound plans launch on torch's current stream at each call"""
+        if self._bound is None:
+            check(lib().mdt_fx_set_stream(self._h, _stream()))
 
     def set_precision(self, xprec):
         """X-engine precision; the X input's sample format and pitches follow
@@ -357,6 +380,7 @@ class FxPlan:
         """FP8 plans: int32 [N] exponents e_c of the last F-engine run; the
         X-engine quantises e4m3(x*s*2^e_c)."""
         e = torch.empty(self.N, dtype=torch.int32, device="cuda")
+        self._sync_stream()
         check(lib().mdt_fx_get_channel_exponents(self._h, _ptr(e, torch.int32)))
         return e
 
@@ -373,10 +397,15 @@ class FxPlan:
                                      _ptr(delays, torch.float32) if delays is not None else None))
 
     def set_stream(self, stream_handle):
-        check(lib().mdt_fx_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+        """bind the plan to a HIP stream (None: torch's current stream at each
+        call, the default)"""
+        self._bound = stream_handle
+        if stream_handle is not None:
+            check(lib().mdt_fx_set_stream(self._h, ctypes.c_void_p(stream_handle)))
 
     def window(self):
         W = torch.empty((self.T, self.N), dtype=torch.float32, device="cuda")
+        self._sync_stream()
         check(lib().mdt_fx_get_window(self._h, _ptr(W, torch.float32)))
         return W
 
@@ -453,17 +482,20 @@ class FxPlan:
 
     def fengine(self, x8, xin):
         self._check_sizes(x8=x8, xin=xin)
+        self._sync_stream()
         check(lib().mdt_fx_fengine(self._h, _ptr(x8, torch.int8), _ptr(xin)))
         return xin
 
     def xengine(self, xin, vis, acc64=None):
         self._check_sizes(xin=xin, vis=vis, acc64=acc64)
+        self._sync_stream()
         check(lib().mdt_fx_xengine(self._h, _ptr(xin), _ptr(vis, torch.complex64),
                                    _ptr(acc64, torch.complex128) if acc64 is not None else None))
         return vis
 
     def dump(self, x8, xin, vis, acc64=None):
         self._check_sizes(x8=x8, xin=xin, vis=vis, acc64=acc64)
+        self._sync_stream()
         check(lib().mdt_fx_dump(self._h, _ptr(x8, torch.int8), _ptr(xin),
                                 _ptr(vis, torch.complex64),
                                 _ptr(acc64, torch.complex128) if acc64 is not None else None))
@@ -529,8 +561,11 @@ class FxStream:
         check(lib().mdt_stream_prime(self._h, ctypes.c_void_p(h.ctypes.data)))
 
     def submit(self, slot, acc64=None):
+        """queue one dump; it first waits for the work queued so far on
+        torch's current stream (e.g. zeroing acc64)"""
         if acc64 is not None and acc64.numel() < self.plan.N * self.plan.nb:
             raise ValueError("accumulator smaller than N x NN(NN+1)/2")
+        check(lib().mdt_stream_set_caller_stream(self._h, _stream()))
         check(lib().mdt_stream_submit(self._h, slot, _ptr(acc64, torch.complex128) if acc64 is not None else None))
 
     def wait(self, slot):

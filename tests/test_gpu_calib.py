@@ -164,3 +164,30 @@ def test_gain_stage_in_every_xinput_layout(xprec, N_, T_):
         same = e == eref
         assert same.mean() > 0.99
         assert np.array_equal(v[same], vref[same]), layout
+
+
+def test_gain_scale_follows_the_fengine_launch(c5):
+    """FP16 split with gains large enough to change the FP16 pre-scale
+    (|g| = 3): the X-engine divides by the scale the F-engine used, also when
+    the gains are switched off between the F- and X-engine calls, and gains
+    updated in place are picked up at the next launch (no host read-back)"""
+    rng = np.random.default_rng(56)
+    plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
+    s0 = plan.split_scale()
+    g_big = (3.0 * np.exp(1j * rng.uniform(-np.pi, np.pi, NN))).astype(np.complex64)
+    tau = torch.from_numpy(c5["tau"]).cuda()
+    g = torch.from_numpy(c5["g"]).cuda()
+    plan.set_gains(g, tau)
+    g.copy_(torch.from_numpy(g_big).cuda())  # in place, after set_gains
+    xin = plan.fengine(c5["x8"], plan.alloc_xinput())
+    s1 = plan.split_scale()
+    plan.set_gains(None, None)               # between the F- and the X-engine
+    v = _host(plan.xengine(xin, plan.alloc_vis()))
+    assert s1 * 2 <= s0, (s0, s1)            # the gains did change the scale
+    V64 = orc.calib_apply(_host(c5["V"]).astype(np.complex128), g_big.astype(np.complex128),
+                          c5["tau"].astype(np.float64), NN, N)
+    assert _autonorm(v, V64) < 1e-5
+    # gains off again: back to the plan's own scale and the ungained result
+    v0 = _host(plan.dump(c5["x8"], plan.alloc_xinput(), plan.alloc_vis()))
+    assert plan.split_scale() == s0
+    assert _autonorm(v0, _host(c5["V"]).astype(np.complex128)) < 1e-5

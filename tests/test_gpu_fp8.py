@@ -27,24 +27,14 @@ import torch
 
 import mdtfx
 import oracle as orc
+from e4m3 import E4M3_MAX, e4m3_rne  # pinned against torch's float8_e4m3fn (test_oracle.py)
 
 pytestmark = pytest.mark.gpu
-
-E4M3_MAX = 448.0
 
 
 def _host(t):
     torch.cuda.synchronize()
     return t.cpu().numpy()
-
-
-def e4m3_rne(y):
-    """round float64 values to OCP e4m3fn (3 mantissa bits, bias 7, normals
-    down to 2^-6, subnormal step 2^-9), nearest-even, |y| <= 448 assumed"""
-    a = np.abs(y)
-    E = np.floor(np.log2(np.maximum(a, 2.0 ** -6)))
-    q = 2.0 ** (E - 3)
-    return np.sign(y) * np.rint(a / q) * q
 
 
 def _autonorm(v, v64, NN):

@@ -331,3 +331,59 @@ def test_calib_grad_vs_finite_differences():
         tm[a] -= h
         fd = (orc.calib_loss_grad(V, M, g, tp, NN, N)[0] - orc.calib_loss_grad(V, M, g, tm, NN, N)[0]) / (2 * h)
         assert abs(fd - gt[a]) < 1e-6 * max(1.0, abs(gt[a])), (a, fd, gt[a])
+
+
+def test_xengine_f64_branch_pinned(golden):
+    """orc_xengine_full_f64 (the complex128 branch of oracle.xengine_full):
+    on the golden input widened to complex128 it must give the einsum golden
+    output, and on values that FP32 represents it must give bit for bit what
+    the complex64 branch (orc_xengine_full_from_f32) gives; on complex128
+    values that FP32 cannot hold, it must follow a numpy einsum of those
+    values (a branch that rounded its input to FP32 would miss by ~1e-8)."""
+    x = golden["xeng_in"]
+    Nc, Ns, NN = x.shape
+    V = orc.xengine_full(x.astype(np.complex128), NN, Ns, Nc)
+    assert np.max(np.abs(V - golden["xeng_out"])) < 1e-12
+    rng = np.random.default_rng(11)
+    y = _rand_c64(rng, (5, 33, 7))
+    a = orc.xengine_full(y, 7, 33, 5)
+    b = orc.xengine_full(y.astype(np.complex128), 7, 33, 5)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    z = rng.standard_normal((3, 9, 4)) + 1j * rng.standard_normal((3, 9, 4))  # full FP64 mantissas
+    ref = np.einsum("csa,csb->cba", z, np.conj(z)) / 9
+    got = orc.xengine_full(z, 4, 9, 3)
+    assert np.max(np.abs(got - ref)) < 1e-14
+    z32 = orc.xengine_full(z.astype(np.complex64), 4, 9, 3)
+    assert np.max(np.abs(z32 - ref)) > 1e-9  # the pin can tell the two branches apart
+    # alpha is applied: alpha = 2 doubles every entry of the default 1/Ns
+    assert np.allclose(orc.xengine_full(z, 4, 9, 3, alpha=2.0), 18.0 * ref, rtol=0, atol=1e-13)
+
+
+def test_e4m3_quantiser_matches_torch_float8():
+    """the test-side e4m3 quantiser (tests/e4m3.py) against torch's
+    float8_e4m3fn conversion: random values over the whole range, every
+    exactly representable value, every midpoint between neighbours (ties to
+    even), and the subnormals, both signs.  All inputs are float32 values, so
+    torch's float32 -> float8 conversion rounds once, as the quantiser does."""
+    torch = pytest.importorskip("torch")
+    from e4m3 import E4M3_MAX, e4m3_rne
+    # every finite e4m3fn value (254 of the 256 codes are finite, 0x7f/0xff are NaN)
+    codes = torch.arange(256, dtype=torch.int32).to(torch.uint8).view(torch.float8_e4m3fn)
+    vals = codes.to(torch.float32).numpy()
+    vals = np.unique(vals[np.isfinite(vals)]).astype(np.float64)
+    assert vals.max() == E4M3_MAX and vals.min() == -E4M3_MAX and len(vals) == 253  # +-0 merge
+    pos = vals[vals >= 0]
+    mids = (pos[:-1] + pos[1:]) / 2  # exact ties
+    rng = np.random.default_rng(12)
+    rand = np.concatenate([
+        rng.uniform(-E4M3_MAX, E4M3_MAX, 200000),
+        np.exp2(rng.uniform(-12, np.log2(E4M3_MAX), 100000)) * rng.choice([-1, 1], 100000),
+        rng.uniform(-2.0 ** -6, 2.0 ** -6, 10000),  # subnormal range
+    ])
+    x = np.concatenate([vals, mids, -mids, rand, np.nextafter(mids, 0), np.nextafter(mids, 1e9)])
+    x = x[np.abs(x) <= E4M3_MAX].astype(np.float32)
+    want = torch.from_numpy(x).to(torch.float8_e4m3fn).to(torch.float32).numpy().astype(np.float64)
+    got = e4m3_rne(x.astype(np.float64))
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, (x[bad[:5]], got[bad[:5]], want[bad[:5]])
+    assert x.size > 310000
