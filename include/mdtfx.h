@@ -110,8 +110,11 @@ int mdt_pps_batch_i8(int Nchannels, int Ntaps, const float *Window, const int8_t
  * Replaces cufftPlan1d(&plan, Nchannels, CUFFT_C2C, Nspectra*NN),
  * cufftSetStream, cufftExecC2C(plan, in, out, CUFFT_FORWARD)
  * (MDT_OneStation_client.cu:622, 630, 1105).  Batched, unnormalised, natural
- * bin order; in place or out of place.  nx must be a power of two in
- * [8, 16384] (MDT_ERR_UNSUPPORTED otherwise). */
+ * bin order; in place or out of place.  nx: every length the client accepts,
+ * [32, 16384] (MDT_OneStation_client.h:56-57), and powers of two from 8:
+ * powers of two in radix-8 Stockham passes, other 7-smooth lengths
+ * (2^a 3^b 5^c 7^d) in mixed-radix passes, the rest by Bluestein's chirp-z
+ * convolution (MDT_ERR_UNSUPPORTED outside these). */
 typedef struct mdt_fft_plan_s *mdt_fft_plan;
 int mdt_fft_plan_1d(mdt_fft_plan *plan, int nx, int type, int batch);
 int mdt_fft_set_stream(mdt_fft_plan plan, hipStream_t stream);
@@ -206,7 +209,11 @@ int mdt_write_binary(const char *path, const void *vis, int layout, int dtype, i
  * FP32, baseline bl(a,b) = a(a+1)/2 + b for a >= b, V = (1/Ns) sum_s
  * x_a conj(x_b) (the alpha of MDT_OneStation_client.cu:683).  Optionally adds
  * V into an FP64 complex accumulator of the same layout (cross-dump
- * integration).  The plan owns the window and twiddle tables only. */
+ * integration).  The plan owns the window and twiddle tables only.
+ * Nchannels must be a power of two here (MDT_ERR_UNSUPPORTED otherwise: the
+ * fused F-engine's transforms are radix-8 Stockham); other channel counts the
+ * client allows run through the drop-in calls (mdt_pps_batch, the any-length
+ * mdt_fft_exec_c2c, mdt_reorder_foutput, mdt_cgemm_batched). */
 typedef struct mdt_fx_plan_s *mdt_fx_plan;
 int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, int Ntaps,
                        int Nspectra, int window_kind, int xprec);

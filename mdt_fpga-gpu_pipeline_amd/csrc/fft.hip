@@ -7,17 +7,10 @@
 // global memory by the first/last Stockham pass, 64-lane contiguous.
 #include <new>
 
+#include "fft_any.hpp"
 #include "fft_core.hpp"
 
 namespace mdt {
-
-template <int N>
-struct FftCfg {
-  static constexpr int P = fft_points(N);
-  static constexpr int TPB = N / P;
-  static constexpr int FPB = TPB >= 256 ? 1 : 256 / TPB;  // transforms per block
-  static constexpr int THREADS = TPB * FPB;
-};
 
 template <int N, bool INV>
 __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_kernel(const float2 *in, float2 *out,
@@ -81,7 +74,8 @@ struct mdt_fft_plan_s {
   int nx = 0;
   long batch = 0;
   hipStream_t stream = 0;
-  float2 *tw = nullptr;
+  float2 *tw = nullptr;  // powers of two: W_nx^k
+  mdt::FftAny any;       // other lengths: mixed radix or Bluestein tables
 };
 
 using namespace mdt;
@@ -93,14 +87,17 @@ int mdt_fft_plan_1d(mdt_fft_plan *plan, int nx, int type, int batch) {
   *plan = nullptr;
   if (type != MDT_FFT_C2C) return fail(MDT_ERR_UNSUPPORTED, "only C2C transforms are provided");
   if (batch < 1) return fail(MDT_ERR_INVALID_VALUE, "batch=%d must be >= 1", batch);
-  if (!is_pow2(nx) || nx < 8 || nx > 16384)
-    return fail(MDT_ERR_UNSUPPORTED, "FFT length %d is not a power of two in [8, 16384]", nx);
+  const bool pow2 = is_pow2(nx) && nx >= 8 && nx <= 16384;
+  if (!pow2 && (nx < MDT_MIN_CHANNELS || nx > MDT_MAX_CHANNELS))
+    return fail(MDT_ERR_UNSUPPORTED, "FFT length %d: powers of two in [8, 16384], other lengths in [%d, %d]", nx,
+                MDT_MIN_CHANNELS, MDT_MAX_CHANNELS);
   mdt_fft_plan p = new (std::nothrow) mdt_fft_plan_s;
   if (!p) return fail(MDT_ERR_ALLOC, "plan");
   p->nx = nx;
   p->batch = batch;
-  int st = make_twiddles(nx, &p->tw);
+  int st = pow2 ? make_twiddles(nx, &p->tw) : fft_any_create(nx, &p->any);
   if (st != MDT_SUCCESS) {
+    fft_any_destroy(&p->any);
     delete p;
     return st;
   }
@@ -122,6 +119,9 @@ int mdt_fft_exec_c2c(mdt_fft_plan plan, mdt_complex *idata, mdt_complex *odata, 
     return fail(MDT_ERR_INVALID_VALUE, "direction must be -1 (forward) or +1 (inverse)");
   if (((uintptr_t)idata | (uintptr_t)odata) & 7)
     return fail(MDT_ERR_MISALIGNED, "complex buffers must be 8-byte aligned");
+  if (plan->any.kind != 0)
+    return fft_any_exec(plan->any, (const float2 *)idata, (float2 *)odata, plan->batch, direction == MDT_FFT_INVERSE,
+                        plan->stream);
   return fft_dispatch(plan->nx, (const float2 *)idata, (float2 *)odata, plan->batch, plan->tw,
                       direction == MDT_FFT_INVERSE, plan->stream);
 }
@@ -129,6 +129,7 @@ int mdt_fft_exec_c2c(mdt_fft_plan plan, mdt_complex *idata, mdt_complex *odata, 
 int mdt_fft_destroy(mdt_fft_plan plan) {
   if (!plan) return MDT_SUCCESS;
   if (plan->tw) (void)hipFree(plan->tw);
+  fft_any_destroy(&plan->any);
   delete plan;
   return MDT_SUCCESS;
 }

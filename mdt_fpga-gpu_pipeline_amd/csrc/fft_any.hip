@@ -1,0 +1,455 @@
+// fft_any.hip -- a3 for the lengths that are not powers of two: the drop-in
+// cufftPlan1d / cufftExecC2C (MDT_OneStation_client.cu:622, 1105) accepts any
+// Nchannels in [32, 16384] (MDT_OneStation_client.h:56-57).  Unnormalised,
+// natural order, forward sign -1, in place or out of place (see fft_any.hpp).
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "fft_any.hpp"
+
+namespace mdt {
+
+// ------------------------------------------------------------------ mixed radix
+// cos / sin(2 pi j / R), j < R, for the odd radices
+__device__ constexpr float kC3[3] = {1.0f, -0.5f, -0.5f};
+__device__ constexpr float kS3[3] = {0.0f, 0.86602540378443865f, -0.86602540378443865f};
+__device__ constexpr float kC5[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f, -0.80901699437494742f,
+                                     0.30901699437494742f};
+__device__ constexpr float kS5[5] = {0.0f, 0.95105651629515357f, 0.58778525229247313f, -0.58778525229247313f,
+                                     -0.95105651629515357f};
+__device__ constexpr float kC7[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f, -0.90096886790241913f,
+                                     -0.90096886790241913f, -0.22252093395631440f, 0.62348980185873353f};
+__device__ constexpr float kS7[7] = {0.0f, 0.78183148246802981f, 0.97492791218182361f, 0.43388373911755812f,
+                                     -0.43388373911755812f, -0.97492791218182361f, -0.78183148246802981f};
+
+template <int R>
+struct OddTab;
+template <>
+struct OddTab<3> {
+  __device__ static float c(int j) { return kC3[j]; }
+  __device__ static float s(int j) { return kS3[j]; }
+};
+template <>
+struct OddTab<5> {
+  __device__ static float c(int j) { return kC5[j]; }
+  __device__ static float s(int j) { return kS5[j]; }
+};
+template <>
+struct OddTab<7> {
+  __device__ static float c(int j) { return kC7[j]; }
+  __device__ static float s(int j) { return kS7[j]; }
+};
+
+// R-point DFT, R odd, natural order in and out.  Inputs r and R-r pair up:
+// a_m = x_m + x_{R-m}, b_m = x_m - x_{R-m}; then with theta = 2 pi m k / R
+//   y_k = x_0 + sum_m a_m cos(theta) - i sum_m b_m sin(theta),  y_{R-k} = the same with + i
+// (forward; the inverse swaps the signs of the i terms).
+template <int R, bool INV>
+__device__ __forceinline__ void dft_odd(float2 *v) {
+  constexpr int H = (R - 1) / 2;
+  float2 a[H], b[H];
+#pragma unroll
+  for (int m = 1; m <= H; m++) {
+    a[m - 1] = cadd(v[m], v[R - m]);
+    b[m - 1] = csub(v[m], v[R - m]);
+  }
+  float2 y[R];
+  y[0] = v[0];
+#pragma unroll
+  for (int m = 0; m < H; m++) y[0] = cadd(y[0], a[m]);
+#pragma unroll
+  for (int k = 1; k <= H; k++) {
+    float2 A = v[0], B = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 1; m <= H; m++) {
+      const int j = (m * k) % R;
+      const float c = OddTab<R>::c(j), s = OddTab<R>::s(j);
+      A = make_float2(__builtin_fmaf(a[m - 1].x, c, A.x), __builtin_fmaf(a[m - 1].y, c, A.y));
+      B = make_float2(__builtin_fmaf(b[m - 1].x, s, B.x), __builtin_fmaf(b[m - 1].y, s, B.y));
+    }
+    // -i B = (B.y, -B.x)
+    const float2 mi = make_float2(A.x + B.y, A.y - B.x), pi = make_float2(A.x - B.y, A.y + B.x);
+    y[k] = INV ? pi : mi;
+    y[R - k] = INV ? mi : pi;
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = y[r];
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2 *v) {
+  if constexpr (R == 3 || R == 5 || R == 7)
+    dft_odd<R, INV>(v);
+  else
+    Dft<R, INV>::run(v);
+}
+
+// One Stockham pass of radix R at stage NS (the product of the earlier
+// radices) over an N-point vector: butterfly j < N/R reads v_r = x[j + r N/R],
+// multiplies by W_{NS R}^{(j mod NS) r} = W_N^{(j mod NS) r N/(NS R)}, does an
+// R-point DFT and writes y[(j / NS) NS R + j mod NS + r NS].  The thread
+// handles butterflies t, t + TPB, ... (at most QM: TPB >= N/16).  Sources and
+// destinations are global memory (the first pass reads the input row, the
+// last writes the output row, both 64-lane contiguous) or the transform's LDS
+// buffer (padded by lpad).  A pass that reads and writes LDS waits at a
+// barrier between its reads and its writes; a pass that writes LDS waits at
+// one after its writes.
+template <int R, bool INV>
+__device__ __attribute__((noinline)) void mixed_pass(const float2 *gsrc, float2 *gdst, float2 *lds,
+                                           bool from_lds, bool to_lds, bool valid, int N, int NS, int t, int TPB,
+                                           const float2 *__restrict__ tw) {
+  constexpr int QM = (16 + R - 1) / R;
+  const int nb = N / R;
+  float2 v[QM][R];
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int j = t + q * TPB;
+    if (j < nb) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = j + r * nb;
+        v[q][r] = from_lds ? lds[lpad(i)] : (valid ? gsrc[i] : make_float2(0.f, 0.f));
+      }
+    }
+  }
+  if (from_lds && to_lds) __syncthreads();
+  const int tstep = N / (NS * R);
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int j = t + q * TPB;
+    if (j < nb) {
+      const int jm = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          float2 w = tw[jm * r * tstep];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft_any<R, INV>(v[q]);
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int o = base + r * NS;
+        if (to_lds)
+          lds[lpad(o)] = v[q][r];
+        else if (valid)
+          store_stream(gdst + o, v[q][r]);
+      }
+    }
+  }
+  if (to_lds) __syncthreads();
+}
+
+struct MixedArgs {
+  const float2 *in;
+  float2 *out;
+  long batch;
+  const float2 *tw;
+  int N, npass, TPB, FPB, lds_stride;
+  int rad[kMaxPasses];
+};
+
+template <bool INV>
+__global__ __launch_bounds__(1024) void fft_mixed_kernel(MixedArgs a) {
+  extern __shared__ float2 lds_dyn[];
+  const int f = threadIdx.x / a.TPB, t = threadIdx.x % a.TPB;
+  const long row = (long)blockIdx.x * a.FPB + f;
+  const bool valid = row < a.batch;
+  float2 *lds = lds_dyn + f * a.lds_stride;
+  const float2 *src = a.in + (valid ? row : 0) * a.N;
+  float2 *dst = a.out + (valid ? row : 0) * a.N;
+  int NS = 1;
+  for (int p = 0; p < a.npass; p++) {
+    const bool from_lds = p > 0, to_lds = p + 1 < a.npass;
+    switch (a.rad[p]) {
+      case 2: mixed_pass<2, INV>(src, dst, lds, from_lds, to_lds, valid, a.N, NS, t, a.TPB, a.tw); break;
+      case 3: mixed_pass<3, INV>(src, dst, lds, from_lds, to_lds, valid, a.N, NS, t, a.TPB, a.tw); break;
+      case 4: mixed_pass<4, INV>(src, dst, lds, from_lds, to_lds, valid, a.N, NS, t, a.TPB, a.tw); break;
+      case 5: mixed_pass<5, INV>(src, dst, lds, from_lds, to_lds, valid, a.N, NS, t, a.TPB, a.tw); break;
+      case 7: mixed_pass<7, INV>(src, dst, lds, from_lds, to_lds, valid, a.N, NS, t, a.TPB, a.tw); break;
+      default: mixed_pass<8, INV>(src, dst, lds, from_lds, to_lds, valid, a.N, NS, t, a.TPB, a.tw); break;
+    }
+    NS *= a.rad[p];
+  }
+}
+
+// ------------------------------------------------------------------ Bluestein
+// v as a value the compiler cannot see through.  The transforms and table
+// reads of one kernel repeat the same addresses (twiddles, chirps, LDS
+// offsets): laundered per phase, they are recomputed or reloaded where used
+// instead of being kept live across the transforms in between (without this
+// the 16384-point kernels spill).
+template <typename V>
+__device__ __forceinline__ V fresh(V v) {
+  if constexpr (sizeof(V) == 8)
+    asm volatile("" : "+s"(v));
+  else
+    asm volatile("" : "+v"(v));
+  return v;
+}
+
+// X_k = c_k sum_n (x_n c_n) conj(c_{k-n}), c_n = exp(-+ pi i n^2 / N) (nk =
+// (n^2 + k^2 - (k-n)^2) / 2): a circular convolution of a_n = x_n c_n (zero
+// past N) with b_m = conj(c_m) (wrapped, b_{M-m} = b_m) of length M >= 2N-1,
+// done as IFFT_M(FFT_M(a) FFT_M(b)) with FFT_M(b) / M precomputed (filt).
+// The thread holds points t + m TPB of the M-point vector throughout: the
+// power-of-two core returns X[t + m TPB] where it took x[t + m TPB].
+template <int M>
+__global__ __launch_bounds__(FftCfg<M>::THREADS) void fft_bluestein_kernel(const float2 *in, float2 *out, long batch,
+                                                                          int N, const float2 *__restrict__ chirp,
+                                                                          const float2 *__restrict__ filt,
+                                                                          const float2 *__restrict__ twm) {
+  using C = FftCfg<M>;
+  __shared__ float2 lds[C::FPB * lds_entries<M>()];
+  const int t = threadIdx.x % C::TPB;
+  const int f = threadIdx.x / C::TPB;
+  const long row = (long)blockIdx.x * C::FPB + f;
+  const bool valid = row < batch;
+  const float2 *src = in + (valid ? row : 0) * N;
+  float2 x[C::P];
+#pragma unroll
+  for (int m = 0; m < C::P; m++) {
+    const int n = t + m * C::TPB;
+    x[m] = (valid && n < N) ? cmul(src[n], chirp[n]) : make_float2(0.f, 0.f);
+  }
+  float2 *l = lds + f * lds_entries<M>();
+  fft_block<M, C::P, false>(x, t, l, twm);
+#pragma unroll
+  for (int m = 0; m < C::P; m++) x[m] = cmul(x[m], filt[t + m * C::TPB]);
+  fft_block<M, C::P, true>(x, fresh(t), l, fresh(twm));
+  if (valid) {
+    float2 *dst = out + row * N;
+    const float2 *ck = fresh(chirp);
+#pragma unroll
+    for (int m = 0; m < C::P; m++) {
+      const int k = t + m * C::TPB;
+      if (k < N) store_stream(dst + k, cmul(x[m], ck[k]));
+    }
+  }
+}
+
+// N > 8192: M = 32768 = 2H.  With a_n = 0 for n >= N and N <= H,
+//   A_{2k} = FFT_H(a)_k,  A_{2k+1} = FFT_H(a_n W_M^n)_k,
+//   c_k = IFFT_H(C_even)_k + W_M^{-k} IFFT_H(C_odd)_k  (k < H),
+// C = A FFT_M(b) / M.  Two H-point transforms each way; the even half's
+// result stays in registers while the odd half runs.
+constexpr int kSplitH = 16384;
+__global__ __launch_bounds__(FftCfg<kSplitH>::THREADS) void fft_bluestein_split_kernel(
+    const float2 *in, float2 *out, long batch, int N, const float2 *__restrict__ chirp,
+    const float2 *__restrict__ filt, const float2 *__restrict__ twh, const float2 *__restrict__ twm) {
+  using C = FftCfg<kSplitH>;
+  static_assert(C::FPB == 1, "one transform per workgroup");
+  __shared__ float2 lds[lds_entries<kSplitH>()];
+  const int t = threadIdx.x;
+  const long row = blockIdx.x;
+  const float2 *src = in + row * N;
+  float2 e[C::P], x[C::P];
+#pragma unroll
+  for (int m = 0; m < C::P; m++) {
+    const int n = t + m * C::TPB;
+    e[m] = n < N ? cmul(src[n], chirp[n]) : make_float2(0.f, 0.f);
+  }
+  fft_block<kSplitH, C::P, false>(e, t, lds, twh);
+#pragma unroll
+  for (int m = 0; m < C::P; m++) e[m] = cmul(e[m], filt[t + m * C::TPB]);
+  fft_block<kSplitH, C::P, true>(e, fresh(t), lds, fresh(twh));
+  {
+    const int t1 = fresh(t);
+    const float2 *s1 = fresh(src), *c1 = fresh(chirp), *w1 = fresh(twm);
+#pragma unroll
+    for (int m = 0; m < C::P; m++) {
+      const int n = t1 + m * C::TPB;
+      x[m] = n < N ? cmul(cmul(s1[n], c1[n]), w1[n]) : make_float2(0.f, 0.f);
+    }
+  }
+  fft_block<kSplitH, C::P, false>(x, fresh(t), lds, fresh(twh));
+  {
+    const int t1 = fresh(t);
+    const float2 *f1 = fresh(filt) + kSplitH;
+#pragma unroll
+    for (int m = 0; m < C::P; m++) x[m] = cmul(x[m], f1[t1 + m * C::TPB]);
+  }
+  fft_block<kSplitH, C::P, true>(x, fresh(t), lds, fresh(twh));
+  float2 *dst = out + row * N;
+  const int t1 = fresh(t);
+  const float2 *c1 = fresh(chirp), *w1 = fresh(twm);
+#pragma unroll
+  for (int m = 0; m < C::P; m++) {
+    const int k = t1 + m * C::TPB;
+    if (k < N) {
+      const float2 w = w1[k];
+      const float2 o = cmul(x[m], make_float2(w.x, -w.y));  // W_M^{-k}
+      store_stream(dst + k, cmul(cadd(e[m], o), c1[k]));
+    }
+  }
+  (void)batch;
+}
+
+// ------------------------------------------------------------------ host side
+bool fft_factor(int N, int *npass, int *rad) {
+  int n = N, p = 0;
+  while (n % 8 == 0 && p < kMaxPasses) rad[p++] = 8, n /= 8;
+  if (n % 4 == 0 && p < kMaxPasses) rad[p++] = 4, n /= 4;
+  if (n % 2 == 0 && p < kMaxPasses) rad[p++] = 2, n /= 2;
+  for (int r : {3, 5, 7})
+    while (n % r == 0 && p < kMaxPasses) rad[p++] = r, n /= r;
+  *npass = p;
+  return n == 1;
+}
+
+// in-place radix-2 FFT in FP64 (host), sign -1
+static void host_fft(std::vector<std::complex<double>> &a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; i++) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; k++) {
+        const double ang = -2.0 * M_PI * (double)k / (double)len;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+// exp(i pi q / N) for an integer q in [0, 2N), argument reduced to [-1, 1)
+static std::complex<double> expi_pi(long q, int N) {
+  double r = (double)q / (double)N;
+  if (r >= 1.0) r -= 2.0;
+  return std::complex<double>(std::cos(M_PI * r), std::sin(M_PI * r));
+}
+
+static int upload(const std::vector<float2> &h, float2 **d) {
+  if (hipMalloc(d, sizeof(float2) * h.size()) != hipSuccess) return fail(MDT_ERR_ALLOC, "FFT table");
+  if (hipMemcpy(*d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(MDT_ERR_DEVICE, "FFT table upload");
+  return MDT_SUCCESS;
+}
+
+static float2 to_f2(std::complex<double> z) { return make_float2((float)z.real(), (float)z.imag()); }
+
+int fft_any_create(int N, FftAny *p) {
+  *p = FftAny{};
+  p->N = N;
+  if (fft_factor(N, &p->npass, p->rad)) {
+    p->kind = FFT_ANY_MIXED;
+    p->TPB = (N + 7) / 8 <= 1024 ? (N + 7) / 8 : (N + 15) / 16;
+    p->FPB = p->TPB >= 256 ? 1 : 256 / p->TPB;
+    const int stride = N + (N >> 4) + 1;  // lpad(N - 1) < stride
+    p->lds_bytes = sizeof(float2) * (size_t)stride * p->FPB;
+    for (int inv = 0; inv < 2; inv++) {
+      const void *k = inv ? (const void *)fft_mixed_kernel<true> : (const void *)fft_mixed_kernel<false>;
+      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes) != hipSuccess)
+        return fail(MDT_ERR_DEVICE, "LDS of %zu bytes for the %d-point FFT", p->lds_bytes, N);
+    }
+    return make_twiddles(N, &p->tw);
+  }
+  int M = 1;
+  while (M < 2 * N - 1) M <<= 1;
+  p->M = M;
+  p->kind = M <= kSplitH ? FFT_ANY_BLUESTEIN : FFT_ANY_BLUESTEIN_SPLIT;
+  int st = make_twiddles(p->kind == FFT_ANY_BLUESTEIN ? M : kSplitH, &p->tw);
+  for (int inv = 0; inv < 2 && st == MDT_SUCCESS; inv++) {
+    const double sg = inv ? 1.0 : -1.0;  // the transform's sign
+    std::vector<std::complex<double>> c(N), b(M, 0.0);
+    std::vector<float2> hc(N);
+    for (long n = 0; n < N; n++) {
+      const std::complex<double> e = expi_pi((n * n) % (2L * N), N);
+      c[n] = sg < 0 ? std::conj(e) : e;
+      hc[n] = to_f2(c[n]);
+    }
+    for (long m = 0; m < N; m++) {
+      b[m] = std::conj(c[m]);
+      if (m > 0) b[M - m] = std::conj(c[m]);
+    }
+    host_fft(b);
+    std::vector<float2> hf(M);
+    if (p->kind == FFT_ANY_BLUESTEIN) {
+      for (int k = 0; k < M; k++) hf[k] = to_f2(b[k] / (double)M);
+    } else {  // even bins, then odd bins
+      for (int k = 0; k < kSplitH; k++) {
+        hf[k] = to_f2(b[2 * k] / (double)M);
+        hf[kSplitH + k] = to_f2(b[2 * k + 1] / (double)M);
+      }
+    }
+    st = upload(hc, &p->chirp[inv]);
+    if (st == MDT_SUCCESS) st = upload(hf, &p->filt[inv]);
+  }
+  if (st == MDT_SUCCESS && p->kind == FFT_ANY_BLUESTEIN_SPLIT) {
+    std::vector<float2> h(kSplitH);
+    for (long n = 0; n < kSplitH; n++) h[n] = to_f2(std::conj(expi_pi(n, kSplitH)));  // exp(-2 pi i n / M)
+    st = upload(h, &p->twm);
+  }
+  return st;
+}
+
+void fft_any_destroy(FftAny *p) {
+  float2 *ptrs[] = {p->tw, p->chirp[0], p->chirp[1], p->filt[0], p->filt[1], p->twm};
+  for (float2 *q : ptrs)
+    if (q) (void)hipFree(q);
+  *p = FftAny{};
+}
+
+template <int M>
+static int launch_blue(const FftAny &p, const float2 *in, float2 *out, long batch, bool inv, hipStream_t st) {
+  using C = FftCfg<M>;
+  const long blocks = (batch + C::FPB - 1) / C::FPB;
+  MDT_TRY(check_grid(blocks, C::THREADS, "FFT batch"));
+  hipLaunchKernelGGL(fft_bluestein_kernel<M>, dim3((unsigned)blocks), dim3(C::THREADS), 0, st, in, out, batch, p.N,
+                     p.chirp[inv], p.filt[inv], p.tw);
+  return check_launch("fft_bluestein_kernel");
+}
+
+int fft_any_exec(const FftAny &p, const float2 *in, float2 *out, long batch, bool inv, hipStream_t st) {
+  if (p.kind == FFT_ANY_MIXED) {
+    MixedArgs a{};
+    a.in = in;
+    a.out = out;
+    a.batch = batch;
+    a.tw = p.tw;
+    a.N = p.N;
+    a.npass = p.npass;
+    a.TPB = p.TPB;
+    a.FPB = p.FPB;
+    a.lds_stride = p.N + (p.N >> 4) + 1;
+    for (int i = 0; i < p.npass; i++) a.rad[i] = p.rad[i];
+    const long blocks = (batch + p.FPB - 1) / p.FPB;
+    const int threads = p.TPB * p.FPB;
+    MDT_TRY(check_grid(blocks, threads, "FFT batch"));
+    if (inv)
+      hipLaunchKernelGGL(fft_mixed_kernel<true>, dim3((unsigned)blocks), dim3(threads), p.lds_bytes, st, a);
+    else
+      hipLaunchKernelGGL(fft_mixed_kernel<false>, dim3((unsigned)blocks), dim3(threads), p.lds_bytes, st, a);
+    return check_launch("fft_mixed_kernel");
+  }
+  const int d = inv ? 1 : 0;
+  if (p.kind == FFT_ANY_BLUESTEIN_SPLIT) {
+    MDT_TRY(check_grid(batch, FftCfg<kSplitH>::THREADS, "FFT batch"));
+    hipLaunchKernelGGL(fft_bluestein_split_kernel, dim3((unsigned)batch), dim3(FftCfg<kSplitH>::THREADS), 0, st, in,
+                       out, batch, p.N, p.chirp[d], p.filt[d], p.tw, p.twm);
+    return check_launch("fft_bluestein_split_kernel");
+  }
+  switch (p.M) {
+    case 64: return launch_blue<64>(p, in, out, batch, d, st);
+    case 128: return launch_blue<128>(p, in, out, batch, d, st);
+    case 256: return launch_blue<256>(p, in, out, batch, d, st);
+    case 512: return launch_blue<512>(p, in, out, batch, d, st);
+    case 1024: return launch_blue<1024>(p, in, out, batch, d, st);
+    case 2048: return launch_blue<2048>(p, in, out, batch, d, st);
+    case 4096: return launch_blue<4096>(p, in, out, batch, d, st);
+    case 8192: return launch_blue<8192>(p, in, out, batch, d, st);
+    case 16384: return launch_blue<16384>(p, in, out, batch, d, st);
+    default: return fail(MDT_ERR_UNSUPPORTED, "Bluestein length %d", p.M);
+  }
+}
+
+}  // namespace mdt

@@ -32,6 +32,16 @@ constexpr int full_passes(int n, int p) {
 // points per thread for an N-point transform
 constexpr int fft_points(int N) { return N <= 4096 ? 8 : 16; }
 
+// launch shape of the standalone N-point transforms: FPB transforms of TPB
+// threads (P points each) per workgroup
+template <int N>
+struct FftCfg {
+  static constexpr int P = fft_points(N);
+  static constexpr int TPB = N / P;
+  static constexpr int FPB = TPB >= 256 ? 1 : 256 / TPB;  // transforms per block
+  static constexpr int THREADS = TPB * FPB;
+};
+
 // W_16^k constants, k = 0..7: cos(2 pi k / 16), sin(2 pi k / 16)
 __device__ constexpr float kCos16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f,
                                         0.38268343236508978f, 0.0f, -0.38268343236508978f,

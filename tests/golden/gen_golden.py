@@ -6,7 +6,8 @@ is independent of the oracle's code:
     (periodic, sym=False) -- the same windowed-sinc that
     MDT_device_functions.cu:52-88 evaluates;
   * fft: numpy.fft.fft (pocketfft, FP64), the contract of the cufftExecC2C
-    forward call at MDT_OneStation_client.cu:1105;
+    forward call at MDT_OneStation_client.cu:1105, at powers of two and at
+    lengths that are not (1000, 1536, 3000, 4099);
   * xengine: numpy.einsum in FP64 of alpha * A A^H per channel, the contract of
     cublasCgemmBatched(OP_N, OP_C) at MDT_OneStation_client.cu:1128;
   * chain: numpy FP64 FIR (einsum) -> numpy.fft -> einsum on a small int8
@@ -63,6 +64,15 @@ def main():
     out["chain_W"] = W
     out["chain_vis"] = V[:, a, b]
     out["chain_shape"] = np.array([NN, N, T, Ns])
+    # FFT lengths that are not powers of two (mixed radix 2^a 3^b 5^c; the
+    # prime 4099 takes the chirp-z path), from their own generator so the
+    # vectors above stay as they were
+    rng2 = np.random.default_rng(20261204)
+    for N in [1000, 1536, 3000, 4099]:
+        x = rng2.standard_normal((1, N)) + 1j * rng2.standard_normal((1, N))
+        out[f"fft_in_N{N}"] = x
+        out[f"fft_out_N{N}"] = np.fft.fft(x, axis=-1)
+        out[f"ifft_out_N{N}"] = np.fft.ifft(x, axis=-1) * N
     np.savez_compressed(os.path.join(HERE, "golden.npz"), **out)
     print("wrote", os.path.join(HERE, "golden.npz"), sum(v.nbytes for v in out.values()), "bytes raw")
 

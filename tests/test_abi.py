@@ -68,7 +68,10 @@ def test_limits_rejected_before_launch(L):
 def test_fft_plan_rejections(L):
     import mdtfx
     h = ctypes.c_void_p()
-    assert L.mdt_fft_plan_1d(ctypes.byref(h), 1000, mdtfx.FFT_C2C, 4) == mdtfx.ERR_UNSUPPORTED
+    # lengths outside the client's [32, 16384] that are not powers of two >= 8
+    # (1000 is accepted: it needs a GPU to build its tables, tested there)
+    for n in (12, 31, 16385, 100000, 4):
+        assert L.mdt_fft_plan_1d(ctypes.byref(h), n, mdtfx.FFT_C2C, 4) == mdtfx.ERR_UNSUPPORTED, n
     assert L.mdt_fft_plan_1d(ctypes.byref(h), 1024, 0x2a, 4) == mdtfx.ERR_UNSUPPORTED  # R2C
     assert L.mdt_fft_plan_1d(ctypes.byref(h), 1024, mdtfx.FFT_C2C, 0) == mdtfx.ERR_INVALID_VALUE
     assert L.mdt_fft_exec_c2c(None, None, None, -1) == mdtfx.ERR_NULL_POINTER

@@ -107,7 +107,7 @@ def test_pps_linearity():
 
 
 # ----------------------------------------------------------------- FFT
-@pytest.mark.parametrize("N", [8, 32, 1024, 2048, 4096])
+@pytest.mark.parametrize("N", [8, 32, 1024, 2048, 4096, 1000, 1536, 3000, 4099])
 def test_fft_f64_matches_numpy_golden(golden, N):
     x = golden[f"fft_in_N{N}"]
     X = orc.fft_f64(x, -1)
