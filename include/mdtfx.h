@@ -264,6 +264,12 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
  * (FP8 needs Nchannels in [512, 4096] and Ntaps 4 or 8).  Run the F-engine at
  * the precision the X-engine will use. */
 int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
+/* The same F-engine on the reference's float F input, d_FInput [Ns+T-1][NN][N]
+ * complex64 as decoded from the wire (MDT_OneStation_client.cu:898-907, 1022,
+ * 1048): replaces the client's PpS_Batch + cufftExecC2C + ReorderFOutput
+ * (client.cu:1083-1114) in one pass, bit-identical to those three drop-in
+ * calls.  FP32 and FP16-split precisions (MDT_ERR_UNSUPPORTED for FP8). */
+int mdt_fx_fengine_f32(mdt_fx_plan plan, const mdt_complex *InSignal, mdt_complex *XInputNative);
 /* Per-input complex gain g_e and delay tau_e (samples) applied by the F-engine
  * to each channel k: X'_e[k] = g_e exp(-2 pi i k_s tau_e / N) X_e[k], k_s the
  * signed bin.  Device arrays of NN entries, read at every F-engine launch
@@ -342,6 +348,44 @@ int mdt_stream_submit(mdt_stream stream, int slot, double *Acc64);
  * visibilities read; times (optional) gets the dump's phase times */
 int mdt_stream_wait(mdt_stream stream, int slot, mdt_dump_times *times);
 int mdt_stream_destroy(mdt_stream stream);
+
+/* ---------------------------------------------------------------- packet streaming correlator
+ * The client's whole per-dump loop on its own wire data
+ * (MDT_OneStation_client.cu:846-1276): SNAP packets -> the rank-1 decode and
+ * scatter into float stacks (client.cu:846-1005, 898-907) -> d_FInput =
+ * PFB history + stack (client.cu:1022, 1048) -> PpS_Batch / cufftExecC2C /
+ * ReorderFOutput / cublasCgemmBatched (client.cu:1083-1128) -> history shift
+ * (client.cu:1170) -> visibilities to the host (client.cu:1199), in one
+ * process.  Two host slots, each a pinned packet buffer of max_pkts packets
+ * of words_per_pkt uint32 words (the mdt_depacketise format) and a pinned
+ * visibility buffer [Nchannels][NN(NN+1)/2] complex64.  A submit takes the
+ * packets of one dump: they are copied to the device, decoded there into the
+ * slot's float F input (mdt_fx_fengine_f32) and correlated, on three HIP
+ * streams ordered by events (as mdt_stream_*).
+ * Dumps are numbered from 0 by submit order.  Sample i of a packet with
+ * counter cnt has index u = cnt*SperElement + i - (Ntaps-1)*Nchannels (the
+ * client's steady-state index, client.cu:953); dump d takes u in
+ * [d*Ns*N, (d+1)*Ns*N), and dump 0, the first block (client.cu:855), also
+ * u in [-(Ntaps-1)*N, 0), its PFB history; later dumps carry the previous
+ * dump's last Ntaps-1 spectra on the device.  A sample no packet carries is 0
+ * (the client would keep the stack's stale value).  Samples of another dump,
+ * bad headers and short payloads are dropped.  counts (mdt_pstream_wait,
+ * nullable, 3 entries): samples written to the dump's spectra, to its history
+ * spectra (dump 0 only), dropped -- per (packet, sample), as the client's
+ * Nsamples[].  The plan must use the FP32 or FP16-split precision; the stream
+ * binds the plan's stream while it exists.  Needs Nspectra >= Ntaps-1. */
+typedef struct mdt_pstream_s *mdt_pstream;
+int mdt_pstream_create(mdt_pstream *stream, mdt_fx_plan plan, int words_per_pkt, int SperElement, int max_pkts);
+int mdt_pstream_host_packets(mdt_pstream stream, int slot, uint32_t **pkts);
+int mdt_pstream_host_vis(mdt_pstream stream, int slot, mdt_complex **vis);
+/* as mdt_stream_set_caller_stream */
+int mdt_pstream_set_caller_stream(mdt_pstream stream, hipStream_t caller);
+/* the next submit is dump 0 again (a new first block); no dump may be in flight */
+int mdt_pstream_restart(mdt_pstream stream);
+/* Acc64 as in mdt_stream_submit (same ordering rules) */
+int mdt_pstream_submit(mdt_pstream stream, int slot, int npkts, double *Acc64);
+int mdt_pstream_wait(mdt_pstream stream, int slot, mdt_dump_times *times, unsigned long long *counts);
+int mdt_pstream_destroy(mdt_pstream stream);
 
 /* ---------------------------------------------------------------- calibration (a9)
  * No counterpart in the reference.  For raw triangle visibilities V (as from

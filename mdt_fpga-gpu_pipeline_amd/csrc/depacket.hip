@@ -52,18 +52,6 @@ __device__ __forceinline__ SampleDest sample_dest(uint32_t cnt, int i, int S, in
   return d;
 }
 
-// header check: element range and payload size (uniform per packet)
-__device__ __forceinline__ bool header_ok(uint32_t station, uint32_t snap, int S, int NN, int words_per_pkt,
-                                          int *elem0, int *nin) {
-  if (snap >= (uint32_t)kSnapsPerStation || station >= 65536u) return false;
-  int e = 0;
-  for (uint32_t k = 0; k < snap; k++) e += kInputsPerSnap[k];
-  const long e0 = (long)station * kInputsPerStation + e;
-  *nin = kInputsPerSnap[snap];
-  *elem0 = (int)e0;
-  return e0 + *nin <= NN && 4 + 2L * S * *nin <= words_per_pkt;
-}
-
 __global__ void depacket_kernel(const uint32_t *__restrict__ pkts, int npkts, int words_per_pkt, int S,
                                 int NN, int N, int Ns, int T, int first_block, float2 *__restrict__ stacks,
                                 unsigned long long *__restrict__ counts) {

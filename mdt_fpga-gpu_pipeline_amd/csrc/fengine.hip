@@ -6,9 +6,9 @@
 //
 // Workgroup = one input e (or EB inputs for short transforms) x SB consecutive
 // spectra.  Thread t owns FFT points n = t + m*TPB.  For every frame f of the
-// SB + T - 1 needed, the thread loads its P int8 samples once and
-// fma-accumulates them into the SB outputs that use the frame, in tap order
-// (bit-identical to mdt_pps_batch_i8).  Each of the SB spectra is then
+// SB + T - 1 needed, the thread loads its P samples once (int8, or the
+// reference's float2) and fma-accumulates them into the SB outputs that use
+// the frame, in tap order (bit-identical to mdt_pps_batch[_i8]).  Each of the SB spectra is then
 // transformed with the shared Stockham core (bit-identical to
 // mdt_fft_exec_c2c), and the SB x N results are corner-turned through LDS
 // into the X input (rows or channel-pair layout, XGeom in common.hpp): each
@@ -32,7 +32,11 @@ struct FCfg {
   static constexpr int SLICE = FFT_LDS > STAGE_LDS ? FFT_LDS : STAGE_LDS;
 };
 
-template <int N, int T>
+__device__ __forceinline__ float2 fe_sample(char2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ float2 fe_sample(float2 v) { return v; }
+
+// In: char2 (int8 I, Q) or float2 (the reference's float F input)
+template <int N, int T, typename In>
 __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   using C = FCfg<N>;
   constexpr int P = C::P, TPB = C::TPB, SB = C::SB;
@@ -63,7 +67,12 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
     for (int m = 0; m < P; m++) acc[sl][m] = make_float2(0.f, 0.f);
 
   const long spec = (long)a.NN * N;
-  const char2 *src = a.in + (long)s0 * spec + (long)(valid_e ? e : 0) * N + t;
+  const In *in0;
+  if constexpr (sizeof(In) == 8)
+    in0 = a.fin;
+  else
+    in0 = a.in;
+  const In *src = in0 + (long)s0 * spec + (long)(valid_e ? e : 0) * N + t;
   const float *Wt = a.W + t;
 
   if constexpr (T > 0) {
@@ -74,10 +83,7 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
       if (f < scount + T - 1 && valid_e) {
         float2 x[P];
 #pragma unroll
-        for (int m = 0; m < P; m++) {
-          char2 v = src[(long)f * spec + m * TPB];
-          x[m] = make_float2((float)v.x, (float)v.y);
-        }
+        for (int m = 0; m < P; m++) x[m] = fe_sample(src[(long)f * spec + m * TPB]);
 #pragma unroll
         for (int sl = 0; sl < SB; sl++) {
           const int i = f - sl;
@@ -99,10 +105,10 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
         for (int i = 0; i < Tn; i++) {
 #pragma unroll
           for (int m = 0; m < P; m++) {
-            char2 v = src[(long)(sl + i) * spec + m * TPB];
+            const float2 v = fe_sample(src[(long)(sl + i) * spec + m * TPB]);
             const float wv = Wt[(long)i * N + m * TPB];
-            acc[sl][m].x = __builtin_fmaf(wv, (float)v.x, acc[sl][m].x);
-            acc[sl][m].y = __builtin_fmaf(wv, (float)v.y, acc[sl][m].y);
+            acc[sl][m].x = __builtin_fmaf(wv, v.x, acc[sl][m].x);
+            acc[sl][m].y = __builtin_fmaf(wv, v.y, acc[sl][m].y);
           }
         }
       }
@@ -652,21 +658,31 @@ static int launch_persistent(const FArgs &a, hipStream_t st) {
 template <int N>
 static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   using C = FCfg<N>;
+  const bool fin = a.fin != nullptr;
   if constexpr (N >= 512 && N <= 4096) {
-    if (a.T == 4) return launch_persistent<N, 4>(a, st);
-    if (a.T == 8) return launch_persistent<N, 8>(a, st);
+    if (!fin && a.T == 4) return launch_persistent<N, 4>(a, st);
+    if (!fin && a.T == 8) return launch_persistent<N, 8>(a, st);
   }
   if (a.split == 2)
-    return fail(MDT_ERR_UNSUPPORTED, "FP8 X-engine input needs Nchannels in [512, 4096] and Ntaps 4 or 8");
+    return fail(MDT_ERR_UNSUPPORTED,
+                "FP8 X-engine input needs int8 samples, Nchannels in [512, 4096] and Ntaps 4 or 8");
   const long nsb = (a.Ns + C::SB - 1) / C::SB;
   const long neg = (a.NN + C::EB - 1) / C::EB;
   const long total = nsb * neg;
   MDT_TRY(check_grid(total, C::THREADS, "F-engine"));
   dim3 grid((unsigned)total), block(C::THREADS);
-  switch (a.T) {
-    case 4: hipLaunchKernelGGL((fengine_kernel<N, 4>), grid, block, 0, st, a); break;
-    case 8: hipLaunchKernelGGL((fengine_kernel<N, 8>), grid, block, 0, st, a); break;
-    default: hipLaunchKernelGGL((fengine_kernel<N, 0>), grid, block, 0, st, a); break;
+  if (fin) {
+    switch (a.T) {
+      case 4: hipLaunchKernelGGL((fengine_kernel<N, 4, float2>), grid, block, 0, st, a); break;
+      case 8: hipLaunchKernelGGL((fengine_kernel<N, 8, float2>), grid, block, 0, st, a); break;
+      default: hipLaunchKernelGGL((fengine_kernel<N, 0, float2>), grid, block, 0, st, a); break;
+    }
+  } else {
+    switch (a.T) {
+      case 4: hipLaunchKernelGGL((fengine_kernel<N, 4, char2>), grid, block, 0, st, a); break;
+      case 8: hipLaunchKernelGGL((fengine_kernel<N, 8, char2>), grid, block, 0, st, a); break;
+      default: hipLaunchKernelGGL((fengine_kernel<N, 0, char2>), grid, block, 0, st, a); break;
+    }
   }
   return check_launch("fengine_kernel");
 }

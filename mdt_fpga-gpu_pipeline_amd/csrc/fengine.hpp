@@ -7,6 +7,7 @@ namespace mdt {
 
 struct FArgs {
   const char2 *in;      // [Ns+T-1][NN][N] int8 (I, Q)
+  const float2 *fin;    // or the same frames as float2 (the reference's d_FInput); nullptr: int8
   const float *W;       // [T][N] FP32 window
   const float *Wt;      // persistent kernel: the window as [m][i/4][t][4] (arrange_taps)
   const float2 *tw;     // W_N^k twiddles
@@ -52,6 +53,7 @@ __device__ __forceinline__ float2 split_sample(float2 v, float sc) {
   return make_float2(__uint_as_float(w0), __uint_as_float(w1));
 }
 
+// int8 input (a.in) or float2 input (a.fin, the generic kernel only)
 int launch_fengine(const FArgs &a, int N, hipStream_t st);
 // shapes served by the persistent kernel (N in [512, 4096], T in {4, 8})
 bool persistent_shape(int N, int T);

@@ -284,14 +284,17 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window) {
   return MDT_SUCCESS;
 }
 
-int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative) {
+}  // extern "C"
+
+// the F-engine on int8 (in8) or float2 (fin) input
+static int fengine(mdt_fx_plan plan, const int8_t *in8, const mdt_complex *fin, mdt_complex *XInputNative) {
   MDT_REQUIRE_PTR(plan);
-  MDT_REQUIRE_PTR(InSignal);
   MDT_REQUIRE_PTR(XInputNative);
-  if (((uintptr_t)InSignal & 1) || ((uintptr_t)XInputNative & 15))
-    return fail(MDT_ERR_MISALIGNED, "input must be 2-byte, X input 16-byte aligned");
+  if (((uintptr_t)in8 & 1) || ((uintptr_t)fin & 7) || ((uintptr_t)XInputNative & 15))
+    return fail(MDT_ERR_MISALIGNED, "input must be 2-byte (int8) or 8-byte (float2), X input 16-byte aligned");
   FArgs a{};
-  a.in = (const char2 *)InSignal;
+  a.in = (const char2 *)in8;
+  a.fin = (const float2 *)fin;
   a.W = plan->W;
   a.Wt = plan->Wt;
   a.tw = plan->tw;
@@ -318,6 +321,20 @@ int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInput
     MDT_TRY(check_launch("zero_kernel"));
   }
   return launch_fengine(a, plan->N, plan->stream);
+}
+
+extern "C" {
+
+int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative) {
+  MDT_REQUIRE_PTR(InSignal);
+  return fengine(plan, InSignal, nullptr, XInputNative);
+}
+
+int mdt_fx_fengine_f32(mdt_fx_plan plan, const mdt_complex *InSignal, mdt_complex *XInputNative) {
+  MDT_REQUIRE_PTR(InSignal);
+  if (plan && plan->xprec == MDT_XPREC_FP8)
+    return fail(MDT_ERR_UNSUPPORTED, "the FP8 path takes int8 samples");
+  return fengine(plan, nullptr, InSignal, XInputNative);
 }
 
 int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_complex *Vis,

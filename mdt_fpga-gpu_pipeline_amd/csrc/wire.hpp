@@ -25,4 +25,19 @@ constexpr int kInputsPerStation = 38;
 constexpr int kSnapsPerStation = 4;
 __device__ constexpr int kInputsPerSnap[kSnapsPerStation] = {10, 10, 10, 8};
 
+// Packet header check (uniform per packet): the station and SNAP indices
+// give the first element station*38 + sum of the lower SNAPs' inputs
+// (MDT_OneStation_client.cu:877-885) and the input count; the packet must
+// hold S samples of that many inputs and the elements must exist.
+__device__ __forceinline__ bool header_ok(uint32_t station, uint32_t snap, int S, int NN, int words_per_pkt,
+                                          int *elem0, int *nin) {
+  if (snap >= (uint32_t)kSnapsPerStation || station >= 65536u) return false;
+  int e = 0;
+  for (uint32_t k = 0; k < snap; k++) e += kInputsPerSnap[k];
+  const long e0 = (long)station * kInputsPerStation + e;
+  *nin = kInputsPerSnap[snap];
+  *elem0 = (int)e0;
+  return e0 + *nin <= NN && 4 + 2L * S * *nin <= words_per_pkt;
+}
+
 }  // namespace mdt

@@ -110,6 +110,7 @@ _SIGS = {
     "mdt_fx_xinput_pitch": [_VP, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)],
     "mdt_fx_get_window": [_VP, _VP],
     "mdt_fx_fengine": [_VP, _VP, _VP],
+    "mdt_fx_fengine_f32": [_VP, _VP, _VP],
     "mdt_fx_xengine": [_VP, _VP, _VP, _VP],
     "mdt_fx_dump": [_VP, _VP, _VP, _VP, _VP],
     "mdt_fx_plan_destroy": [_VP],
@@ -124,6 +125,14 @@ _SIGS = {
     "mdt_stream_submit": [_VP, _I, _VP],
     "mdt_stream_wait": [_VP, _I, _VP],
     "mdt_stream_destroy": [_VP],
+    "mdt_pstream_create": [ctypes.POINTER(_VP), _VP, _I, _I, _I],
+    "mdt_pstream_host_packets": [_VP, _I, ctypes.POINTER(_VP)],
+    "mdt_pstream_host_vis": [_VP, _I, ctypes.POINTER(_VP)],
+    "mdt_pstream_set_caller_stream": [_VP, _VP],
+    "mdt_pstream_restart": [_VP],
+    "mdt_pstream_submit": [_VP, _I, _I, _VP],
+    "mdt_pstream_wait": [_VP, _I, _VP, _VP],
+    "mdt_pstream_destroy": [_VP],
     "mdt_synth_i8": [ctypes.POINTER(SynthParams), _I, _I, _I, _VP, _VP],
     "mdt_fx_set_gains": [_VP, _VP, _VP],
     "mdt_calib_plan_create": [ctypes.POINTER(_VP), _I, _I],
@@ -486,6 +495,16 @@ class FxPlan:
         check(lib().mdt_fx_fengine(self._h, _ptr(x8, torch.int8), _ptr(xin)))
         return xin
 
+    def fengine_f32(self, fin, xin):
+        """F-engine on the reference's float F input: complex64
+        [Ns+T-1][NN][N] (FP32 and FP16-split plans)"""
+        if fin.numel() < (self.Ns + self.T - 1) * self.NN * self.N:
+            raise ValueError("input buffer smaller than (Ns+T-1) x NN x N complex64 samples")
+        self._check_sizes(xin=xin)
+        self._sync_stream()
+        check(lib().mdt_fx_fengine_f32(self._h, _ptr(fin, torch.complex64), _ptr(xin)))
+        return xin
+
     def xengine(self, xin, vis, acc64=None):
         self._check_sizes(xin=xin, vis=vis, acc64=acc64)
         self._sync_stream()
@@ -580,6 +599,72 @@ class FxStream:
         if self._h:
             self._stacks, self._vis = [], []
             check(lib().mdt_stream_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class PacketStream:
+    """Packet streaming correlator (mdt_pstream_*): two host slots, each a
+    pinned uint32 packet buffer [max_pkts][words] and pinned visibilities
+    [N][NN(NN+1)/2] complex64 (numpy views).  Write a dump's packets into
+    packets(slot), submit(slot, npkts), later wait(slot) -> (phase times,
+    counts) and read vis(slot).  Dumps are numbered by submit order from 0
+    (the first block, which also carries the PFB history)."""
+
+    def __init__(self, plan, words_per_pkt, SperElement, max_pkts):
+        import numpy as np
+        self._h = ctypes.c_void_p()
+        check(lib().mdt_pstream_create(ctypes.byref(self._h), plan._h, words_per_pkt, SperElement, max_pkts))
+        self.plan, self.words, self.max_pkts = plan, words_per_pkt, max_pkts
+        self._pk, self._vis = [], []
+        for slot in range(2):
+            p = ctypes.c_void_p()
+            check(lib().mdt_pstream_host_packets(self._h, slot, ctypes.byref(p)))
+            buf = (ctypes.c_uint32 * (max_pkts * words_per_pkt)).from_address(p.value)
+            buf._owner = self
+            self._pk.append(np.frombuffer(buf, np.uint32).reshape(max_pkts, words_per_pkt))
+            check(lib().mdt_pstream_host_vis(self._h, slot, ctypes.byref(p)))
+            buf = (ctypes.c_float * (2 * plan.N * plan.nb)).from_address(p.value)
+            buf._owner = self
+            self._vis.append(np.frombuffer(buf, np.complex64).reshape(plan.N, plan.nb))
+
+    def packets(self, slot):
+        if not self._h:
+            raise MdtError(ERR_NULL_POINTER, "stream destroyed")
+        return self._pk[slot]
+
+    def vis(self, slot):
+        if not self._h:
+            raise MdtError(ERR_NULL_POINTER, "stream destroyed")
+        return self._vis[slot]
+
+    def submit(self, slot, npkts, acc64=None):
+        if acc64 is not None and acc64.numel() < self.plan.N * self.plan.nb:
+            raise ValueError("accumulator smaller than N x NN(NN+1)/2")
+        check(lib().mdt_pstream_set_caller_stream(self._h, _stream()))
+        check(lib().mdt_pstream_submit(self._h, slot, int(npkts),
+                                       _ptr(acc64, torch.complex128) if acc64 is not None else None))
+
+    def wait(self, slot):
+        """blocks until the slot's dump is done: (phase times in ms, [samples
+        to the dump's spectra, to its history, dropped])"""
+        t = DumpTimes()
+        c = (ctypes.c_ulonglong * 3)()
+        check(lib().mdt_pstream_wait(self._h, slot, ctypes.byref(t), c))
+        return {k: getattr(t, k) for k, _ in DumpTimes._fields_}, [int(v) for v in c]
+
+    def restart(self):
+        check(lib().mdt_pstream_restart(self._h))
+
+    def destroy(self):
+        if self._h:
+            self._pk, self._vis = [], []
+            check(lib().mdt_pstream_destroy(self._h))
             self._h = ctypes.c_void_p()
 
     def __del__(self):
