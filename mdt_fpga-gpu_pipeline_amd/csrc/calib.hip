@@ -34,119 +34,150 @@ struct Kahan {
 
 __device__ __forceinline__ long bl_of(int a, int b) { return (long)a * (a + 1) / 2 + b; }
 
-// G[a][c] = g_a phi_a(c)
-__global__ void calib_gains_kernel(int NN, int N, const float2 *g, const float *tau, float2 *G) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long)NN * N) return;
-  const int a = (int)(i / N), c = (int)(i % N);
-  const float2 ph = delay_phasor(signed_bin(c, N), tau ? tau[a] : 0.f, N);
-  G[i] = cmul(g[a], ph);
+constexpr int CAL_THREADS = 256, CAL_WAVES = 4, CAL_TILE = 64;
+
+// sum over the 64 lanes of a wave by an xor butterfly: every lane ends with
+// the same bits (each step adds the same two values in either order)
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
 }
 
-__global__ void calib_apply_kernel(int NN, int N, long nb, const float2 *G, const float2 *V,
-                                   float2 *Vp) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long)N * nb) return;
-  const int c = (int)(i / nb);
-  const long bl = i % nb;
-  int a = (int)((sqrt(8.0 * (double)bl + 1.0) - 1.0) / 2.0);
-  while ((long)a * (a + 1) / 2 > bl) a--;
-  while ((long)(a + 1) * (a + 2) / 2 <= bl) a++;
-  const int b = (int)(bl - (long)a * (a + 1) / 2);
-  const float2 ga = G[(long)a * N + c], gb = G[(long)b * N + c];
-  Vp[i] = cmul(cmulc(ga, gb), V[i]);
+// G_a(c) = g_a phi_a(c) of every input a for channel c, into a wave's LDS row
+__device__ __forceinline__ void wave_gains(float2 *Gw, int NN, int N, int c, const float2 *g, const float *tau,
+                                           int lane) {
+  const int kk = signed_bin(c, N);
+  for (int a = lane; a < NN; a += 64) Gw[a] = cmul(g[a], delay_phasor(kk, tau ? tau[a] : 0.f, N));
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
-// loss partials: one block per chunk of channels, threads over baselines
-constexpr int CAL_THREADS = 256;
-
-__device__ float block_sum(float v, float *sh) {
-  sh[threadIdx.x] = v;
+// V' = G_a conj(G_b) V for every (channel, baseline): one block per channel,
+// the channel's gains in LDS, threads over its baselines (coalesced)
+__global__ __launch_bounds__(CAL_THREADS) void calib_apply_kernel(int NN, int N, long nb, const float2 *g,
+                                                                  const float *tau, const float2 *V, float2 *Vp) {
+  extern __shared__ float2 Gs[];
+  const int c = blockIdx.x;
+  const int kk = signed_bin(c, N);
+  for (int a = threadIdx.x; a < NN; a += CAL_THREADS) Gs[a] = cmul(g[a], delay_phasor(kk, tau ? tau[a] : 0.f, N));
   __syncthreads();
-  for (int s = CAL_THREADS / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
+  for (long bl = threadIdx.x; bl < nb; bl += CAL_THREADS) {
+    int a = (int)((sqrt(8.0 * (double)bl + 1.0) - 1.0) / 2.0);
+    while ((long)a * (a + 1) / 2 > bl) a--;
+    while ((long)(a + 1) * (a + 2) / 2 <= bl) a++;
+    const int b = (int)(bl - (long)a * (a + 1) / 2);
+    const long i = (long)c * nb + bl;
+    Vp[i] = cmul(cmulc(Gs[a], Gs[b]), V[i]);
   }
-  const float r = sh[0];
-  __syncthreads();
-  return r;
 }
 
-__global__ __launch_bounds__(CAL_THREADS) void calib_loss_kernel(int NN, int N, long nb, int cpb,
-                                                                 const float2 *G, const float2 *V,
-                                                                 const float2 *M, float *partial) {
-  __shared__ float sh[CAL_THREADS];
-  Kahan acc;
-  const int c0 = blockIdx.x * cpb;
-  for (int c = c0; c < min(N, c0 + cpb); c++) {
-    for (int a = 0; a < NN; a++) {
-      const float2 ga = G[(long)a * N + c];
-      for (int b = threadIdx.x; b <= a; b += CAL_THREADS) {
-        const long i = (long)c * nb + bl_of(a, b);
-        const float2 vp = cmul(cmulc(ga, G[(long)b * N + c]), V[i]);
-        const float rr = vp.x - M[i].x, ri = vp.y - M[i].y;
-        acc.add(__builtin_fmaf(rr, rr, ri * ri));
+// Loss and adjoint partials.  Block (channel chunk, input tile of 64): wave w
+// takes channels c0 + w, c0 + w + 4, ... of the chunk; lane l is input
+// z = 64 tile + l.  Per channel, with the channel's gains G in the wave's LDS
+// row and R = V' - M:
+//   rows:    for each input zz of the tile, its row of baselines (zz, p <= zz)
+//            (contiguous, lanes over p): |R|^2 into the loss, and
+//            R G_p conj(V), Im(conj(R) V') summed over the row for input zz;
+//   columns: for each input p >= the tile's first, its row's baselines
+//            (p, z) with z in the tile (contiguous, lane z): conj(R) G_p V
+//            and -Im(conj(R) V') for input z.
+// Every baseline enters the loss once (rows partition the triangle) and the
+// gradient of each of its two inputs once; the diagonal (z, z) enters both
+// sums, as its two terms.  Lane z then adds 2 S conj(phi_z(c)) and 2 f_c T
+// to its channel sums.  Fixed order throughout: compensated per-lane sums, a
+// butterfly over each row, the four waves combined in order; partials per
+// (input, chunk) and per block, summed in order by calib_finish_kernel.
+template <bool GRAD>
+__global__ __launch_bounds__(CAL_THREADS) void calib_lg_kernel(int NN, int N, long nb, int nchunk,
+                                                               const float2 *g, const float *tau,
+                                                               const float2 *V, const float2 *M,
+                                                               float *gpart, float *lpart) {
+  extern __shared__ float2 Gs[];  // [CAL_WAVES][NN]
+  __shared__ float red[CAL_WAVES][4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int z0 = blockIdx.y * CAL_TILE, z = z0 + lane;
+  const int zend = min(z0 + CAL_TILE, NN);
+  const int cpc = (N + nchunk - 1) / nchunk;
+  const int c0 = blockIdx.x * cpc, c1 = min(N, c0 + cpc);
+  float2 *Gw = Gs + (long)w * NN;
+  Kahan sgr, sgi, stt, sl;
+  for (int c = c0 + w; c < c1; c += CAL_WAVES) {
+    wave_gains(Gw, NN, N, c, g, tau, lane);
+    const long cb = (long)c * nb;
+    Kahan ar, ai, at;  // lane z's row sums (set by the butterflies)
+    for (int zz = z0; zz < zend; zz++) {
+      const float2 Gzz = Gw[zz];
+      const long rb = cb + bl_of(zz, 0);
+      Kahan rr, ri, rt;
+      for (int p = lane; p <= zz; p += 64) {
+        const float2 v = V[rb + p], m = M[rb + p];
+        const float2 Gp = Gw[p];
+        const float2 vp = cmul(cmulc(Gzz, Gp), v);
+        const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
+        sl.add(__builtin_fmaf(r.x, r.x, r.y * r.y));
+        if constexpr (GRAD) {
+          const float2 t = cmulc(cmul(r, Gp), v);
+          rr.add(t.x);
+          ri.add(t.y);
+          if (p != zz) rt.add(__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+        }
+      }
+      if constexpr (GRAD) {
+        const float sr = wave_sum(rr.value()), si = wave_sum(ri.value()), st = wave_sum(rt.value());
+        if (lane == zz - z0) {
+          ar.add(sr);
+          ai.add(si);
+          at.add(st);
+        }
       }
     }
-  }
-  const float s = block_sum(acc.value(), sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-// gradient partials: block (channel chunk, z); one thread per channel,
-// looping over the partner input p; fixed-order tree reduction over the chunk
-__global__ __launch_bounds__(CAL_THREADS) void calib_grad_kernel(int NN, int N, long nb, int nchunk,
-                                                                 const float2 *G, const float *tau,
-                                                                 const float2 *V, const float2 *M,
-                                                                 float *partial) {
-  __shared__ float sh[CAL_THREADS];
-  const int z = blockIdx.y;
-  const int c = blockIdx.x * CAL_THREADS + threadIdx.x;
-  float cr = 0.f, ci = 0.f, ct = 0.f;
-  if (c < N) {
-    const float2 Gz = G[(long)z * N + c];
-    const int kk = signed_bin(c, N);
-    const float f = 6.283185307179586f * (float)kk / (float)N;
-    Kahan sr, si, st;
-    for (int p = 0; p < NN; p++) {
-      const float2 Gp = G[(long)p * N + c];
-      if (p <= z) {  // baseline (a = z, b = p): term R G_p conj(V)
-        const long i = (long)c * nb + bl_of(z, p);
-        const float2 v = V[i], m = M[i];
-        const float2 vp = cmul(cmulc(Gz, Gp), v);
-        const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
-        const float2 t = cmulc(cmul(r, Gp), v);
-        sr.add(t.x);
-        si.add(t.y);
-        if (p != z) st.add(__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));  // Im(conj(R) V')
+    if constexpr (GRAD) {
+      const float2 Gz = z < NN ? Gw[z] : make_float2(0.f, 0.f);
+      for (int p = z0; p < NN; p++) {
+        if (z <= p) {
+          const float2 Gp = Gw[p];
+          const long i = cb + bl_of(p, z);
+          const float2 v = V[i], m = M[i];
+          const float2 vp = cmul(cmulc(Gp, Gz), v);
+          const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
+          const float2 t = cmul(make_float2(r.x, -r.y), cmul(Gp, v));
+          ar.add(t.x);
+          ai.add(t.y);
+          if (p != z) at.add(-__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+        }
       }
-      if (p >= z) {  // baseline (a = p, b = z): term conj(R) G_p V
-        const long i = (long)c * nb + bl_of(p, z);
-        const float2 v = V[i], m = M[i];
-        const float2 vp = cmul(cmulc(Gp, Gz), v);
-        const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
-        const float2 t = cmul(make_float2(r.x, -r.y), cmul(Gp, v));
-        sr.add(t.x);
-        si.add(t.y);
-        if (p != z) st.add(-__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+      if (z < NN) {
+        const int kk = signed_bin(c, N);
+        const float f = 6.283185307179586f * (float)kk / (float)N;
+        const float2 ph = delay_phasor(kk, tau ? tau[z] : 0.f, N);
+        const float2 t = cmulc(make_float2(ar.value(), ai.value()), ph);  // S conj(phi)
+        sgr.add(2.0f * t.x);
+        sgi.add(2.0f * t.y);
+        stt.add(2.0f * f * at.value());
       }
     }
-    // grad_g[z] gets 2 conj(phi_z(c)) S(c); tau gradient 2 f (sum)
-    const float2 ph = delay_phasor(kk, tau ? tau[z] : 0.f, N);
-    const float2 S = make_float2(sr.value(), si.value());
-    const float2 t = cmulc(S, ph);  // S conj(phi)
-    cr = 2.0f * t.x;
-    ci = 2.0f * t.y;
-    ct = 2.0f * f * st.value();
+    // the next channel rewrites this wave's gains after every lane read them
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
-  const float r0 = block_sum(cr, sh);
-  const float r1 = block_sum(ci, sh);
-  const float r2 = block_sum(ct, sh);
-  if (threadIdx.x == 0) {
-    float *o = partial + ((long)z * nchunk + blockIdx.x) * 3;
-    o[0] = r0;
-    o[1] = r1;
-    o[2] = r2;
+  red[w][0][lane] = sgr.value();
+  red[w][1][lane] = sgi.value();
+  red[w][2][lane] = stt.value();
+  red[w][3][lane] = sl.value();
+  __syncthreads();
+  if (w == 0) {
+    Kahan q[4];
+    for (int ww = 0; ww < CAL_WAVES; ww++)
+      for (int k = 0; k < 4; k++) q[k].add(red[ww][k][lane]);
+    if (GRAD && z < NN) {
+      float *o = gpart + ((long)z * nchunk + blockIdx.x) * 3;
+      o[0] = q[0].value();
+      o[1] = q[1].value();
+      o[2] = q[2].value();
+    }
+    const float ls = wave_sum(q[3].value());
+    if (lane == 0) lpart[(long)blockIdx.x * gridDim.y + blockIdx.y] = ls;
   }
 }
 
@@ -174,18 +205,73 @@ __global__ void calib_finish_kernel(int NN, int nchunk, int nlossblk, const floa
   }
 }
 
+// launch geometry of the loss/adjoint for NN inputs and N channels
+struct CalGeom {
+  int nchunk, ntile;
+  long nb;
+  size_t lds;
+};
+constexpr int CAL_MAX_CHUNKS = 256;
+__host__ inline CalGeom cal_geom(int NN, int N) {
+  CalGeom q;
+  q.nchunk = (N + 3) / 4 < CAL_MAX_CHUNKS ? (N + 3) / 4 : CAL_MAX_CHUNKS;
+  q.ntile = (NN + CAL_TILE - 1) / CAL_TILE;
+  q.nb = (long)NN * (NN + 1) / 2;
+  q.lds = sizeof(float2) * (size_t)CAL_WAVES * NN;
+  return q;
+}
+// workspace in floats: gradient partials [NN][nchunk][3], then loss partials [nchunk][ntile]
+__host__ inline size_t cal_scratch_floats(int NN, int N) {
+  const CalGeom q = cal_geom(NN, N);
+  return 3 * (size_t)NN * q.nchunk + (size_t)q.nchunk * q.ntile;
+}
+
+// the calibration loss / adjoint on a given workspace (no allocation)
+int calib_loss_grad_run(int NN, int N, const float2 *V, const float2 *M, const float2 *g, const float *tau,
+                        float *loss, float2 *grad_g, float *grad_tau, float *work, hipStream_t st) {
+  const CalGeom q = cal_geom(NN, N);
+  float *gpart = work, *lpart = work + 3 * (size_t)NN * q.nchunk;
+  static bool attr_set = false;  // LDS above 64 KiB for the largest arrays
+  if (!attr_set) {
+    if (hipFuncSetAttribute((const void *)calib_lg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(float2) * CAL_WAVES * MDT_MAX_INPUTS)) != hipSuccess ||
+        hipFuncSetAttribute((const void *)calib_lg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(float2) * CAL_WAVES * MDT_MAX_INPUTS)) != hipSuccess)
+      return fail(MDT_ERR_DEVICE, "LDS size of the calibration kernels");
+    attr_set = true;
+  }
+  const dim3 grid(q.nchunk, q.ntile);
+  if (grad_g)
+    hipLaunchKernelGGL(calib_lg_kernel<true>, grid, dim3(CAL_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V, M,
+                       gpart, lpart);
+  else
+    hipLaunchKernelGGL(calib_lg_kernel<false>, grid, dim3(CAL_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V,
+                       M, gpart, lpart);
+  MDT_TRY(check_launch("calib_lg_kernel"));
+  hipLaunchKernelGGL(calib_finish_kernel, dim3(1), dim3(256), 0, st, NN, q.nchunk, q.nchunk * q.ntile, gpart, lpart,
+                     loss, grad_g, grad_tau);
+  return check_launch("calib_finish_kernel");
+}
+
 }  // namespace mdt
 
 struct mdt_calib_plan_s {
   int NN = 0, N = 0;
   hipStream_t stream = 0;
-  float2 *G = nullptr;    // [NN][N] combined gains
-  float *gpart = nullptr; // [NN][nchunk][3]
-  float *lpart = nullptr; // [nlossblk]
-  int nchunk = 0, nlossblk = 0, cpb = 0;
+  float *work = nullptr;  // loss / adjoint partials (cal_scratch_floats)
 };
 
 using namespace mdt;
+
+static int check_calib_args(const mdt_complex *V, const mdt_complex *M, const mdt_complex *g, float *loss,
+                            mdt_complex *grad_g, float *grad_tau) {
+  MDT_REQUIRE_PTR(V);
+  MDT_REQUIRE_PTR(M);
+  MDT_REQUIRE_PTR(g);
+  if (!loss && !grad_g) return fail(MDT_ERR_NULL_POINTER, "nothing to compute (loss and grad_g NULL)");
+  if (grad_tau && !grad_g) return fail(MDT_ERR_NULL_POINTER, "grad_tau needs grad_g");
+  return MDT_SUCCESS;
+}
 
 extern "C" {
 
@@ -198,14 +284,8 @@ int mdt_calib_plan_create(mdt_calib_plan *plan, int NpolsxNelements, int Nchanne
   if (!p) return fail(MDT_ERR_ALLOC, "calibration plan");
   p->NN = NpolsxNelements;
   p->N = Nchannels;
-  p->nchunk = (Nchannels + CAL_THREADS - 1) / CAL_THREADS;
-  p->cpb = 4;
-  p->nlossblk = (Nchannels + p->cpb - 1) / p->cpb;
-  bool ok = hipMalloc(&p->G, sizeof(float2) * (size_t)p->NN * p->N) == hipSuccess &&
-            hipMalloc(&p->gpart, sizeof(float) * 3 * (size_t)p->NN * p->nchunk) == hipSuccess &&
-            hipMalloc(&p->lpart, sizeof(float) * (size_t)p->nlossblk) == hipSuccess;
-  if (!ok) {
-    mdt_calib_plan_destroy(p);
+  if (hipMalloc(&p->work, sizeof(float) * cal_scratch_floats(p->NN, p->N)) != hipSuccess) {
+    delete p;
     return fail(MDT_ERR_ALLOC, "calibration workspace");
   }
   *plan = p;
@@ -218,24 +298,15 @@ int mdt_calib_set_stream(mdt_calib_plan plan, hipStream_t stream) {
   return MDT_SUCCESS;
 }
 
-static int make_G(mdt_calib_plan p, const mdt_complex *g, const float *tau) {
-  const long n = (long)p->NN * p->N;
-  hipLaunchKernelGGL(calib_gains_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, p->NN,
-                     p->N, (const float2 *)g, tau, p->G);
-  return check_launch("calib_gains_kernel");
-}
-
 int mdt_calib_apply(mdt_calib_plan plan, const mdt_complex *V, const mdt_complex *g, const float *tau,
                     mdt_complex *Vp) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(V);
   MDT_REQUIRE_PTR(g);
   MDT_REQUIRE_PTR(Vp);
-  MDT_TRY(make_G(plan, g, tau));
   const long nb = (long)plan->NN * (plan->NN + 1) / 2;
-  const long n = nb * plan->N;
-  hipLaunchKernelGGL(calib_apply_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, plan->stream,
-                     plan->NN, plan->N, nb, plan->G, (const float2 *)V, (float2 *)Vp);
+  hipLaunchKernelGGL(calib_apply_kernel, dim3(plan->N), dim3(CAL_THREADS), sizeof(float2) * plan->NN, plan->stream,
+                     plan->NN, plan->N, nb, (const float2 *)g, tau, (const float2 *)V, (float2 *)Vp);
   return check_launch("calib_apply_kernel");
 }
 
@@ -243,35 +314,14 @@ int mdt_calib_loss_grad(mdt_calib_plan plan, const mdt_complex *V, const mdt_com
                         const mdt_complex *g, const float *tau, float *loss, mdt_complex *grad_g,
                         float *grad_tau) {
   MDT_REQUIRE_PTR(plan);
-  MDT_REQUIRE_PTR(V);
-  MDT_REQUIRE_PTR(M);
-  MDT_REQUIRE_PTR(g);
-  if (!loss && !grad_g) return fail(MDT_ERR_NULL_POINTER, "nothing to compute (loss and grad_g NULL)");
-  if (grad_tau && !grad_g) return fail(MDT_ERR_NULL_POINTER, "grad_tau needs grad_g");
-  MDT_TRY(make_G(plan, g, tau));
-  const long nb = (long)plan->NN * (plan->NN + 1) / 2;
-  if (loss) {
-    hipLaunchKernelGGL(calib_loss_kernel, dim3(plan->nlossblk), dim3(CAL_THREADS), 0, plan->stream,
-                       plan->NN, plan->N, nb, plan->cpb, plan->G, (const float2 *)V,
-                       (const float2 *)M, plan->lpart);
-    MDT_TRY(check_launch("calib_loss_kernel"));
-  }
-  if (grad_g) {
-    hipLaunchKernelGGL(calib_grad_kernel, dim3(plan->nchunk, plan->NN), dim3(CAL_THREADS), 0,
-                       plan->stream, plan->NN, plan->N, nb, plan->nchunk, plan->G, tau,
-                       (const float2 *)V, (const float2 *)M, plan->gpart);
-    MDT_TRY(check_launch("calib_grad_kernel"));
-  }
-  hipLaunchKernelGGL(calib_finish_kernel, dim3(1), dim3(256), 0, plan->stream, plan->NN, plan->nchunk,
-                     plan->nlossblk, plan->gpart, plan->lpart, loss, (float2 *)grad_g, grad_tau);
-  return check_launch("calib_finish_kernel");
+  MDT_TRY(check_calib_args(V, M, g, loss, grad_g, grad_tau));
+  return calib_loss_grad_run(plan->NN, plan->N, (const float2 *)V, (const float2 *)M, (const float2 *)g, tau, loss,
+                             (float2 *)grad_g, grad_tau, plan->work, plan->stream);
 }
 
 int mdt_calib_plan_destroy(mdt_calib_plan plan) {
   if (!plan) return MDT_SUCCESS;
-  if (plan->G) (void)hipFree(plan->G);
-  if (plan->gpart) (void)hipFree(plan->gpart);
-  if (plan->lpart) (void)hipFree(plan->lpart);
+  if (plan->work) (void)hipFree(plan->work);
   delete plan;
   return MDT_SUCCESS;
 }

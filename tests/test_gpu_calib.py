@@ -191,3 +191,29 @@ def test_gain_scale_follows_the_fengine_launch(c5):
     v0 = _host(plan.dump(c5["x8"], plan.alloc_xinput(), plan.alloc_vis()))
     assert plan.split_scale() == s0
     assert _autonorm(v0, _host(c5["V"]).astype(np.complex128)) < 1e-5
+
+
+@pytest.mark.parametrize("NN_,N_", [(150, 64), (3, 40)])
+def test_adjoint_tiles_and_determinism(NN_, N_):
+    """inputs over several 64-input tiles (and a tiny array): loss and adjoint
+    on random visibilities against the FP64 oracle, and bit-identical when
+    run twice (fixed-order reductions)"""
+    rng = np.random.default_rng(NN_)
+    nb = NN_ * (NN_ + 1) // 2
+    V = (rng.standard_normal((N_, nb)) + 1j * rng.standard_normal((N_, nb))).astype(np.complex64)
+    M = (rng.standard_normal((N_, nb)) + 1j * rng.standard_normal((N_, nb))).astype(np.complex64)
+    g = (rng.uniform(0.8, 1.2, NN_) * np.exp(1j * rng.uniform(-np.pi, np.pi, NN_))).astype(np.complex64)
+    tau = rng.uniform(-4, 4, NN_).astype(np.float32)
+    cal = CalibPlan(NN_, N_)
+    args = [torch.from_numpy(a).cuda() for a in (V, M, g, tau)]
+    l1, g1, t1 = (_host(x) for x in cal.loss_grad(*args))
+    l2, g2, t2 = (_host(x) for x in cal.loss_grad(*args))
+    assert l1 == l2 and np.array_equal(g1, g2) and np.array_equal(t1, t2)
+    L64, gg64, gt64 = orc.calib_loss_grad(V.astype(np.complex128), M.astype(np.complex128),
+                                          g.astype(np.complex128), tau.astype(np.float64), NN_, N_)
+    assert abs(float(l1) - L64) / L64 < 1e-6
+    assert np.max(np.abs(g1 - gg64)) / np.max(np.abs(gg64)) < 1e-5
+    assert np.max(np.abs(t1 - gt64)) / np.max(np.abs(gt64)) < 1e-5
+    Vp = _host(cal.apply(args[0], args[2], args[3]))
+    Vp64 = orc.calib_apply(V.astype(np.complex128), g.astype(np.complex128), tau.astype(np.float64), NN_, N_)
+    assert np.max(np.abs(Vp - Vp64)) / np.max(np.abs(Vp64)) < 1e-6
