@@ -404,6 +404,20 @@ int mdt_calib_loss_grad(mdt_calib_plan plan, const mdt_complex *V, const mdt_com
                         const mdt_complex *g, const float *tau, float *loss, mdt_complex *grad_g,
                         float *grad_tau);
 int mdt_calib_plan_destroy(mdt_calib_plan plan);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mdt_calib_loss_grad without a plan, computing the same
+ * loss and adjoint with the same kernels (bit-identical results) on a
+ * workspace the library holds in a static device array of 12.06 MiB per GPU
+ * (3 162 112 floats: the partial sums of any shape within the limits,
+ * 3 <= NpolsxNelements <= 4096, 32 <= Nchannels <= 16384; larger needs are
+ * refused with MDT_ERR_INVALID_VALUE).  It allocates nothing, ever.  Calls on
+ * one GPU share its workspace: calls that overlap on one GPU (on different
+ * streams, or from several threads) race; the workspace needs no clearing
+ * (every partial is written before it is read). */
+int mdt_calib_loss_grad_noalloc(int NpolsxNelements, int Nchannels, const mdt_complex *V, const mdt_complex *M,
+                                const mdt_complex *g, const float *tau, float *loss, mdt_complex *grad_g,
+                                float *grad_tau, hipStream_t stream);
 
 /* ---------------------------------------------------------------- synthetic input
  * Deterministic counter-based generator of int8 (I, Q) samples for the

@@ -15,6 +15,7 @@
 // with R = V' - M.  All sums are FP32 with Neumaier compensation and a fixed
 // reduction order (deterministic); the phasor argument k*tau is reduced modulo
 // N exactly before sincospi.
+#include <mutex>
 #include <new>
 
 #include "calib.hpp"
@@ -255,6 +256,38 @@ int calib_loss_grad_run(int NN, int N, const float2 *V, const float2 *M, const f
 
 }  // namespace mdt
 
+// Workspace of the allocation-free entry point (mdt_calib_loss_grad_noalloc):
+// sized for the largest shape the limits allow (4096 inputs, 256 channel
+// chunks, 64 input tiles), 3 162 112 floats = 12.06 MiB per device, held for
+// the life of the process.  Host code never uses the array's name as an
+// address: each device's copy is reached through the pointer that
+// hipGetSymbolAddress returns with that device current.
+constexpr size_t kCalScratchFloats =
+    3 * (size_t)MDT_MAX_INPUTS * mdt::CAL_MAX_CHUNKS + (size_t)mdt::CAL_MAX_CHUNKS * (MDT_MAX_INPUTS / mdt::CAL_TILE);
+static __device__ __align__(256) float scratch[kCalScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch[kMaxDevices] = {};  // per device: the address of its scratch
+
+int device_scratch(float **ptr) {
+  int dev = 0;
+  MDT_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return mdt::fail(MDT_ERR_DEVICE, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev] == nullptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return mdt::fail(MDT_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = (float *)p;
+  }
+  *ptr = g_scratch[dev];
+  return MDT_SUCCESS;
+}
+}  // namespace
+
 struct mdt_calib_plan_s {
   int NN = 0, N = 0;
   hipStream_t stream = 0;
@@ -317,6 +350,22 @@ int mdt_calib_loss_grad(mdt_calib_plan plan, const mdt_complex *V, const mdt_com
   MDT_TRY(check_calib_args(V, M, g, loss, grad_g, grad_tau));
   return calib_loss_grad_run(plan->NN, plan->N, (const float2 *)V, (const float2 *)M, (const float2 *)g, tau, loss,
                              (float2 *)grad_g, grad_tau, plan->work, plan->stream);
+}
+
+int mdt_calib_loss_grad_noalloc(int NpolsxNelements, int Nchannels, const mdt_complex *V, const mdt_complex *M,
+                                const mdt_complex *g, const float *tau, float *loss, mdt_complex *grad_g,
+                                float *grad_tau, hipStream_t stream) {
+  MDT_TRY(check_inputs(NpolsxNelements));
+  MDT_TRY(check_channels(Nchannels));
+  MDT_TRY(check_calib_args(V, M, g, loss, grad_g, grad_tau));
+  const size_t need = cal_scratch_floats(NpolsxNelements, Nchannels);
+  if (need > kCalScratchFloats)
+    return fail(MDT_ERR_INVALID_VALUE, "%d inputs x %d channels need %zu scratch floats, the maximum is %zu",
+                NpolsxNelements, Nchannels, need, kCalScratchFloats);
+  float *work = nullptr;
+  MDT_TRY(device_scratch(&work));
+  return calib_loss_grad_run(NpolsxNelements, Nchannels, (const float2 *)V, (const float2 *)M, (const float2 *)g, tau,
+                             loss, (float2 *)grad_g, grad_tau, work, stream);
 }
 
 int mdt_calib_plan_destroy(mdt_calib_plan plan) {

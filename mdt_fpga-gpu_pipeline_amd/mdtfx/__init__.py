@@ -140,6 +140,7 @@ _SIGS = {
     "mdt_calib_apply": [_VP, _VP, _VP, _VP, _VP],
     "mdt_calib_loss_grad": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     "mdt_calib_plan_destroy": [_VP],
+    "mdt_calib_loss_grad_noalloc": [_I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
 }
 
 
