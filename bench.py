@@ -10,10 +10,12 @@ X-engine for the metric's comparison.
 
 Prints ONE JSON line.  value = Msamples/s/input of the FP16-split path =
 Nspectra * Nchannels / (seconds per dump) / 1e6.  Beside it, each skippable:
-the host-inclusive streaming rate (--no-stream), the CPU port and the FP64
-error (--no-cpu), the 512-input FP8 wide-band shape (--no-fp8) and the
-reference's drop-in call sequence at the MDT station's shape (--no-station;
---station-only runs that leg alone).
+the host-inclusive streaming rates, int8 stacks and the reference's SNAP
+packets (--no-stream), the CPU port and the FP64 error (--no-cpu), the
+512-input FP8 wide-band shape (--no-fp8), the reference's drop-in call
+sequence at the MDT station's shape with the FFT at lengths that are not
+powers of two (--no-station; --station-only runs that leg alone) and the
+gain/delay calibration shape (--no-calib).
 """
 import argparse
 import json
@@ -158,7 +160,7 @@ def stream_leg(torch, mdtfx, x8, dumps):
             "stack_bytes": stack_bytes}
 
 
-def station_leg(np, torch, mdtfx, steps, warmup):
+def station_leg(np, torch, mdtfx, steps, warmup, err_channels=0):
     """The MDT station shape (40 inputs, 4-tap PFB, 2048 channels, 4096
     spectra per dump) on the drop-in calls, issued in the reference's rank-2
     order per dump (MDT_OneStation_client.cu:1170 history carry, then 1083
@@ -166,7 +168,8 @@ def station_leg(np, torch, mdtfx, steps, warmup):
     ReorderFOutput, 1128 cublasCgemmBatched over the full NN x NN with
     alpha = 1/Ns), every call bracketed by HIP events on the stream it runs
     on; beside it the native fused path on the same samples.  Per-call
-    rooflines use each call's minimum bytes (or the GEMM's flops)."""
+    rooflines use each call's minimum bytes (or the GEMM's flops); FP64 error
+    of the drop-in and native visibilities on err_channels channels."""
     NN2, N2, T2, NS2 = 40, 2048, 4, 4096
     log(f"bench: drop-in station leg NN={NN2} N={N2} T={T2} Ns={NS2}")
     x8 = mdtfx.synth_i8(NN2, N2, NS2 + T2 - 1, seed=SEED, sigma=16.0, rho=RHO, max_delay=4)
@@ -215,7 +218,7 @@ def station_leg(np, torch, mdtfx, steps, warmup):
             r.update({"flops": flops, "TFLOP/s": flops / (ms * 1e-3) / 1e12,
                       "frac_fp32_mfma": flops / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS})
         per_call[name] = r
-    native = {}
+    native, native_vis = {}, {}
     for label, xprec in (("fp32", mdtfx.XPREC_FP32), ("fp16_split", mdtfx.XPREC_FP16_SPLIT)):
         plan = mdtfx.FxPlan(NN2, N2, T2, NS2, mdtfx.WINDOW_BH4, xprec)
         plan.set_stream(stream)
@@ -224,12 +227,204 @@ def station_leg(np, torch, mdtfx, steps, warmup):
         w_ms, f_ms, x_ms = time_path(torch, mdtfx, plan, x8, xin, vis, None, steps, warmup)
         native[label] = {"value": NS2 * N2 / (w_ms * 1e-3) / 1e6, "ms_per_dump": w_ms, "fengine_ms": f_ms,
                          "xengine_ms": x_ms}
+        native_vis[label] = vis.cpu().numpy()
         del plan, xin, vis
     out = {"workload": "MDT station: 40 inputs, 4-tap PFB (BH4), 2048-pt FFT, 4096 spectra/dump; the "
                        "reference's four drop-in calls + history carry per dump on the float F input",
            "value": NS2 * N2 / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_dump": wall,
            "steps": steps, "calls": per_call, "native_fused": native}
+    if err_channels:
+        # the drop-in dump read the F input after its history carry: its first
+        # T-1 spectra are the last T-1 of the buffer
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        xh = x8.cpu().numpy()
+        xd = xh.copy()
+        xd[:T2 - 1] = xh[NS2:NS2 + T2 - 1]
+        chans = np.unique(np.linspace(0, N2 - 1, err_channels).astype(int))
+        Wh = W.cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        v64d = orc.fx_chain_f64(Wh, xd, NN2, N2, T2, NS2, chans)
+        v64n = orc.fx_chain_f64(Wh, xh, NN2, N2, T2, NS2, chans)
+        t64 = time.perf_counter() - t0
+        a, b = np.tril_indices(NN2)
+        vd = XOut.cpu().numpy()[chans][:, b, a]  # column-major [c][col b][row a] -> triangle
+        out["max_rel_err_vs_fp64"] = {
+            "dropin_cgemm": _autonorm_err(np, orc, vd, v64d, NN2),
+            "native_fp32": _autonorm_err(np, orc, native_vis["fp32"][chans], v64n, NN2),
+            "native_fp16_split": _autonorm_err(np, orc, native_vis["fp16_split"][chans], v64n, NN2),
+            "channels_checked": len(chans), "oracle_seconds": round(t64, 1),
+            "definition": "max over (channel, a>=b) of |V - V64| / sqrt(V64_aa V64_bb)"}
     del fft, xe, f_in, F, XIn, XOut, x8
+    torch.cuda.empty_cache()
+    # the drop-in FFT at channel counts that are not powers of two, same
+    # number of rows: mixed radix (1000 = 2^3 5^3) and Bluestein (1009, prime)
+    out["fft_non_pow2"] = {}
+    rows = NS2 * NN2
+    for n, how in ((1000, "mixed radix 8 x 5 x 5 x 5"), (1009, "Bluestein, 2048-point convolution")):
+        fp = mdtfx.FftPlan(n, rows)
+        fp.set_stream(stream)
+        buf = torch.randn((rows, n), dtype=torch.complex64, device="cuda")
+        for _ in range(warmup):
+            fp.exec_c2c(buf, buf, mdtfx.FFT_FORWARD)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fp.exec_c2c(buf, buf, mdtfx.FFT_FORWARD)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        nbytes = 16 * rows * n
+        out["fft_non_pow2"][str(n)] = {"algorithm": how, "rows": rows, "ms": ms, "algorithmic_bytes": nbytes,
+                                       "GB/s": nbytes / (ms * 1e-3) / 1e9,
+                                       "frac_hbm": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        fp.destroy()
+        del buf
+    torch.cuda.empty_cache()
+    return out
+
+
+def _autonorm_err(np, orc, v, v64, nn):
+    a, b = np.tril_indices(nn)
+    diag = np.array([orc.tri_index(i, i) for i in range(nn)])
+    auto = np.real(v64[:, diag])
+    return float(np.max(np.abs(v - v64) / np.sqrt(auto[:, a] * auto[:, b])))
+
+
+def calib_leg(np, torch, mdtfx, steps, warmup, with_err):
+    """The calibration shape: 64 inputs, 4-tap PFB, 1024-pt FFT, 512 spectra;
+    per-input gains |g| ~ U(0.8, 1.2), phase ~ U(-pi, pi), delays ~ U(-4, 4)
+    samples applied in the F-engine; FP32 X-engine; the loss sum |V' - M|^2
+    and its hand-written adjoint (mdt_calib_loss_grad) on the raw
+    visibilities.  Each kernel timed with HIP events on its stream."""
+    from mdtfx.calib import CalibPlan
+
+    NN5, N5, T5, NS5 = 64, 1024, 4, 512
+    log(f"bench: calibration leg NN={NN5} N={N5} T={T5} Ns={NS5}")
+    rng = np.random.default_rng(55)
+    plan = mdtfx.FxPlan(NN5, N5, T5, NS5, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP32)
+    x8 = mdtfx.synth_i8(NN5, N5, NS5 + T5 - 1, seed=SEED, sigma=16.0, rho=0.5, max_delay=4)
+    xin, V = plan.alloc_xinput(), plan.alloc_vis()
+    plan.dump(x8, xin, V)
+    g_true = (rng.uniform(0.8, 1.2, NN5) * np.exp(1j * rng.uniform(-np.pi, np.pi, NN5))).astype(np.complex64)
+    tau_true = rng.uniform(-4, 4, NN5).astype(np.float32)
+    g = torch.from_numpy((g_true * (1 + 0.05 * (rng.standard_normal(NN5) + 1j * rng.standard_normal(NN5))))
+                         .astype(np.complex64)).cuda()
+    tau = torch.from_numpy((tau_true + 0.1 * rng.standard_normal(NN5)).astype(np.float32)).cuda()
+    cal = CalibPlan(NN5, N5)
+    M = cal.apply(V, torch.from_numpy(g_true).cuda(), torch.from_numpy(tau_true).cuda())
+    plan.set_gains(g, tau)
+    vis = plan.alloc_vis()
+    for _ in range(warmup):
+        plan.fengine(x8, xin)
+        plan.xengine(xin, vis)
+        cal.loss_grad(V, M, g, tau)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record()
+        plan.fengine(x8, xin)
+        ev[k][1].record()
+        plan.xengine(xin, vis)
+        ev[k][2].record()
+        loss, gg, gt = cal.loss_grad(V, M, g, tau)
+        ev[k][3].record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ms = [sum(e[i].elapsed_time(e[i + 1]) for e in ev) / steps for i in range(3)]
+    nb5 = NN5 * (NN5 + 1) // 2
+    f_bytes = NN5 * (NS5 + T5 - 1) * N5 * 2 + T5 * N5 * 4 + NN5 * NS5 * N5 * 8
+    x_flops = 8.0 * N5 * NS5 * nb5
+    c_bytes = 2 * 2 * N5 * nb5 * 8  # V and M, read by the row and the column sweeps
+    out = {"workload": "calibration: 64 inputs, 4-tap PFB, 1024-pt FFT, 512 spectra; gains and delays in the "
+                       "F-engine, FP32 X-engine, FP32 loss with the hand-written adjoint",
+           "value": NS5 * N5 / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_step": wall,
+           "steps": steps,
+           "kernels": {
+               "fengine_gained": {"ms": ms[0], "algorithmic_bytes": f_bytes, "GB/s": f_bytes / (ms[0] * 1e-3) / 1e9,
+                                  "frac_hbm": f_bytes / (ms[0] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+               "xengine_fp32": {"ms": ms[1], "useful_flops": x_flops,
+                                "useful_TFLOP/s": x_flops / (ms[1] * 1e-3) / 1e12,
+                                "frac_fp32_mfma": x_flops / (ms[1] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
+               "calib_loss_grad": {"ms": ms[2], "algorithmic_bytes": c_bytes,
+                                   "GB/s": c_bytes / (ms[2] * 1e-3) / 1e9,
+                                   "frac_hbm": c_bytes / (ms[2] * 1e-3) / 1e9 / HBM_PEAK_GBS}}}
+    if with_err:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        V64, M64 = V.cpu().numpy().astype(np.complex128), M.cpu().numpy().astype(np.complex128)
+        L64, gg64, gt64 = orc.calib_loss_grad(V64, M64, g.cpu().numpy().astype(np.complex128),
+                                              tau.cpu().numpy().astype(np.float64), NN5, N5)
+        out["adjoint_vs_fp64"] = {
+            "loss_rel": abs(float(loss.cpu()) - L64) / L64,
+            "grad_gain_rel": float(np.max(np.abs(gg.cpu().numpy() - gg64)) / np.max(np.abs(gg64))),
+            "grad_delay_rel": float(np.max(np.abs(gt.cpu().numpy() - gt64)) / np.max(np.abs(gt64))),
+            "definition": "max |GPU - FP64 analytic| / max |FP64| (orc_calib_loss_grad_f64 on the same V, M, g, tau)"}
+    del plan, cal, x8, xin, V, M, vis
+    torch.cuda.empty_cache()
+    return out
+
+
+def packet_stream_leg(np, torch, mdtfx, dumps):
+    """The client's loop on its own wire data at the shipped station's wire
+    shape (one station: 38 inputs from 4 SNAPs, 2048 channels, 4-tap PFB,
+    4096 spectra per dump; 32 samples per element per packet): each dump's
+    1 048 576 SNAP packets (Fix_32_19 word pairs, 2.7 GB) from pinned host
+    memory over PCIe, decoded on the device into the float F input behind
+    the history, F-engine on the float input, FP32 X-engine, visibilities
+    back.  Packet payloads are built once from int8 samples; each dump only
+    renumbers the packet counters."""
+    NN6, N6, T6, NS6, S = 38, 2048, 4, 4096, 32
+    words = 4 + 2 * S * 10
+    log(f"bench: packet stream leg NN={NN6} N={N6} T={T6} Ns={NS6} S={S}")
+    x8 = mdtfx.synth_i8(NN6, N6, NS6, seed=SEED, sigma=16.0, rho=RHO, max_delay=4).cpu().numpy()
+    per_dump, begins = NS6 * N6, (T6 - 1) * N6
+    ncnt = per_dump // S
+    pk = np.zeros((4 * ncnt, words), np.uint32)
+    X = x8.reshape(NS6, NN6, N6 // S, S, 2)
+    for sn, (nin, e0) in enumerate(zip((10, 10, 10, 8), (0, 10, 20, 30))):
+        v = X[:, e0:e0 + nin].transpose(0, 2, 3, 1, 4).reshape(ncnt, S * nin * 2)
+        rows = pk[sn * ncnt:(sn + 1) * ncnt]
+        rows[:, 1] = sn
+        rows[:, 4:4 + 2 * S * nin] = ((v.astype(np.int32) << 19).view(np.uint32))
+    del X, x8
+    plan = mdtfx.FxPlan(NN6, N6, T6, NS6, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP32)
+    ps = mdtfx.PacketStream(plan, words, S, len(pk))
+    for slot in range(2):
+        ps.packets(slot)[:] = pk
+    cnt = np.tile(np.arange(ncnt, dtype=np.uint32), 4)
+    del pk
+    torch.cuda.synchronize()
+
+    def run(n, d0):
+        times = []
+        for d in range(d0, d0 + n):
+            b = d % 2
+            if d >= d0 + 2:
+                times.append(ps.wait(b))
+            ps.packets(b)[:, 2] = cnt + np.uint32((d * per_dump + begins) // S)
+            ps.submit(b, ps.max_pkts)
+        for d in range(max(d0, d0 + n - 2), d0 + n):
+            times.append(ps.wait(d % 2))
+        return times
+
+    run(2, 0)
+    t0 = time.perf_counter()
+    times = run(dumps, 2)
+    wall = (time.perf_counter() - t0) * 1e3 / dumps
+    ps.destroy()
+    mean = {k: sum(t[0][k] for t in times) / len(times) for k in times[0][0]}
+    pkt_bytes = 4 * ncnt * words * 4
+    out = {"workload": "MDT station wire format: 38 inputs (4 SNAPs), 4-tap PFB, 2048-pt FFT, 4096 spectra/dump, "
+                       "32 samples per element per packet, FP32 X-engine",
+           "value": NS6 * N6 / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_dump": wall,
+           "dumps": dumps, "phase_ms": mean, "packet_bytes": pkt_bytes,
+           "h2d_GB/s": pkt_bytes / (mean["h2d_ms"] * 1e-3) / 1e9,
+           "counts_last_dump": times[-1][1]}
+    del plan
     torch.cuda.empty_cache()
     return out
 
@@ -245,6 +440,13 @@ def load_traffic():
 
 
 def _cpu_model():
+    import subprocess
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -264,17 +466,25 @@ def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, er
 
     W = plan.window().cpu().numpy()
     x_host = x8.cpu().numpy()
+    L, build = orc.lib_native()  # this host's own -march=native build
     nthreads = orc.num_threads()
     sub = np.ascontiguousarray(x_host[: sample_spectra + T - 1])
-    orc.fx_chain_f32(W, np.ascontiguousarray(x_host[: 4 + T - 1]), NN, N, T, 4, nthreads)  # warm-up
-    t0 = time.perf_counter()
-    orc.fx_chain_f32(W, sub, NN, N, T, sample_spectra, nthreads)
-    dt = time.perf_counter() - t0
-    # one core, on a sample 1/16 the size
+    orc.fx_chain_f32_with(L, W, np.ascontiguousarray(x_host[: 4 + T - 1]), NN, N, T, 4, nthreads)  # warm-up
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        orc.fx_chain_f32_with(L, W, sub, NN, N, T, sample_spectra, nthreads)
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
+    # one core, on a sample 1/16 the size, median of 3
     s1 = max(2, sample_spectra // 16)
-    t0 = time.perf_counter()
-    orc.fx_chain_f32(W, np.ascontiguousarray(x_host[: s1 + T - 1]), NN, N, T, s1, 1)
-    dt1 = time.perf_counter() - t0
+    sub1 = np.ascontiguousarray(x_host[: s1 + T - 1])
+    runs1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.fx_chain_f32_with(L, W, sub1, NN, N, T, s1, 1)
+        runs1.append(time.perf_counter() - t0)
+    dt1 = float(np.median(runs1))
     cpu = {
         "value": sample_spectra * N / dt / 1e6,
         "unit": "Msamples/s/input",
@@ -282,9 +492,11 @@ def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, er
         "kind": "port",
         "sample": f"FP32 CPU restatement of the chain (fmaf FIR, radix-2 FFT, corner turn, "
                   f"FP32 triangle correlation) on all {NN} inputs x {N} channels, first "
-                  f"{sample_spectra} of {NS} spectra of the same dump; {dt:.1f} s",
-        "single_core": {"value": s1 * N / dt1 / 1e6, "sample_spectra": s1, "seconds": round(dt1, 1)},
-        "host": {"cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count()},
+                  f"{sample_spectra} of {NS} spectra of the same dump; median of 5 runs of {dt:.2f} s "
+                  f"after one warm-up; build: {build}",
+        "runs_s": [round(r, 3) for r in runs],
+        "single_core": {"value": s1 * N / dt1 / 1e6, "sample_spectra": s1, "seconds_median_of_3": round(dt1, 2)},
+        "host": {"cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(), "omp_threads": nthreads},
     }
     chans = np.unique(np.linspace(0, N - 1, err_channels).astype(int))
     t0 = time.perf_counter()
@@ -313,12 +525,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-spectra", type=int, default=2048)
-    ap.add_argument("--err-channels", type=int, default=8)
+    ap.add_argument("--cpu-spectra", type=int, default=512)
+    ap.add_argument("--err-channels", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 error legs")
     ap.add_argument("--no-fp8", action="store_true", help="skip the wide-band FP8 leg")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-inclusive streaming leg")
     ap.add_argument("--no-station", action="store_true", help="skip the drop-in station leg")
+    ap.add_argument("--no-calib", action="store_true", help="skip the calibration leg")
     ap.add_argument("--station-only", action="store_true",
                     help="only run the drop-in station leg and print its JSON (for per-kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -345,7 +558,7 @@ def main():
         print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
         return
     if args.station_only:
-        print(json.dumps(station_leg(np, torch, mdtfx, args.steps, args.warmup)), flush=True)
+        print(json.dumps(station_leg(np, torch, mdtfx, args.steps, args.warmup, 0)), flush=True)
         return
     stream = torch.cuda.current_stream().cuda_stream
     log(f"bench: scaled array NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
@@ -434,7 +647,12 @@ def main():
         out["fp8_wideband"] = fp8_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), stream,
                                 4, not args.no_cpu)
     if not args.no_station:
-        out["station_dropin"] = station_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2))
+        out["station_dropin"] = station_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2),
+                                            0 if args.no_cpu else args.err_channels)
+    if not args.no_calib:
+        out["calibration"] = calib_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), not args.no_cpu)
+    if not args.no_stream:
+        out["packet_stream_host_inclusive"] = packet_stream_leg(np, torch, mdtfx, 4)
     print(json.dumps(out), flush=True)
 
 

@@ -13,6 +13,8 @@
 // mdt_fft_exec_c2c), and the SB x N results are corner-turned through LDS
 // into the X input (rows or channel-pair layout, XGeom in common.hpp): each
 // store instruction writes runs of consecutive spectra of one (channel, input).
+#include <atomic>
+
 #include "calib.hpp"
 #include "fengine.hpp"
 #include "fft_core.hpp"
@@ -614,13 +616,24 @@ static int launch_persistent_k(const FArgs &a, hipStream_t st) {
   using C = PCfg<N, SPLIT, GAIN>;
   static_assert((C::MR * C::TPB * C::CPR) % C::THREADS == 0, "store chunks must divide among the threads");
   const long nitems = (long)a.NN * ((a.Ns + C::SPAN - 1) / C::SPAN);
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SPLIT, GAIN, XL>,
-                                                   C::THREADS, 0) != hipSuccess)
-    return fail(MDT_ERR_DEVICE, "device attributes for the F-engine grid");
-  long grid = (long)cus * (per_cu > 0 ? per_cu : 1);
+  // resident workgroups per device (CUs x workgroups per CU), queried once per
+  // device for this kernel and kept
+  constexpr int kDevs = 64;
+  static std::atomic<int> resident[kDevs];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevs)
+    return fail(MDT_ERR_DEVICE, "current device for the F-engine grid");
+  int full = resident[dev].load(std::memory_order_relaxed);
+  if (full == 0) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fengine_persistent_kernel<N, T, SPLIT, GAIN, XL>,
+                                                     C::THREADS, 0) != hipSuccess)
+      return fail(MDT_ERR_DEVICE, "device attributes for the F-engine grid");
+    full = cus * (per_cu > 0 ? per_cu : 1);
+    resident[dev].store(full, std::memory_order_relaxed);
+  }
+  long grid = full;
   if (grid > nitems) grid = nitems;
   hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SPLIT, GAIN, XL>), dim3((unsigned)grid), dim3(C::THREADS), 0,
                      st, a, nitems);

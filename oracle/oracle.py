@@ -27,6 +27,41 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+_native = None
+
+
+def lib_native():
+    """The same C source built for this host's own CPU (gcc -O3 -march=native
+    -fopenmp -ffp-contract=off), into a temporary directory: bench.py's CPU
+    baseline times this build.  Falls back to the portable prebuilt library
+    (returned with build = "prebuilt -march=x86-64-v3") when no compiler runs
+    here.  Returns (library, build description)."""
+    global _native
+    if _native is None:
+        import tempfile
+        flags = ["-O3", "-march=native", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11"]
+        out = os.path.join(tempfile.mkdtemp(prefix="mdt_oracle_native_"), "liboracle_native.so")
+        try:
+            subprocess.run(["gcc", *flags, "-shared", "-o", out, os.path.join(_HERE, "mdt_oracle.c"), "-lm"],
+                           check=True, capture_output=True, timeout=120)
+            L = ctypes.CDLL(out)
+            _declare(L)
+            _native = (L, "gcc " + " ".join(flags))
+        except (OSError, subprocess.SubprocessError):
+            _native = (lib(), "prebuilt -march=x86-64-v3")
+    return _native
+
+
+def fx_chain_f32_with(L, W, x8, NN, N, T, Ns, nthreads=0):
+    """orc_fx_chain_f32 through a given build of the oracle (lib_native)"""
+    W = _c(W, np.float32)
+    x8 = _c(x8, np.int8)
+    nb = NN * (NN + 1) // 2
+    vis = np.empty((N, nb), np.complex64)
+    L.orc_fx_chain_f32(NN, N, T, Ns, _p(W), _p(x8), _p(vis), int(nthreads))
+    return vis
+
+
 def lib():
     global _lib
     if _lib is None:
