@@ -73,7 +73,8 @@ enum mdt_status {
 /* X-engine precisions of the native path */
 #define MDT_XPREC_FP32 0       /* v_mfma_f32_32x32x2f32 */
 #define MDT_XPREC_FP16_SPLIT 1 /* hi+lo FP16, three v_mfma_f32_32x32x16_f16 products */
-#define MDT_XPREC_FP8 2        /* e4m3 with per-channel power-of-two RMS scaling, v_mfma_scale 32x32x64 */
+#define MDT_XPREC_FP8 2        /* e4m3 on v_mfma_scale 32x32x64: block exponents (512, 1024 channels) or
+                                  per-channel power-of-two RMS scaling (2048, 4096 channels) */
 
 const char *mdt_last_error_string(void);
 int mdt_version(int *version);
@@ -233,8 +234,8 @@ int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes);
  * = NN*row_pitch, block_pitch = ceil(N/2)*chan_pitch: a block of spectra of
  * channels 2j and 2j+1 of one input is one 128-byte piece.
  * MDT_XIN_BLOCKS: chan_group = 1, block = the spectra in 128 bytes (16 or
- * 32 samples), row_pitch = block, chan_pitch = NN*block, block_pitch =
- * N*chan_pitch.
+ * 32 samples; the block-scaled FP8 format: 32 samples of 2 bytes), row_pitch
+ * = block, chan_pitch = NN*block, block_pitch = N*chan_pitch.
  * MDT_XIN_ROWS: chan_group = 1, block_pitch = 0 (block = 2^30 > s): element
  * k*chan_pitch + e*row_pitch + s, row_pitch = Ns rounded up to even +
  * row_pad, chan_pitch = NN*row_pitch + chan_pad (FP8 path: both rounded up
@@ -242,7 +243,9 @@ int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes);
  * channels off power-of-two strides; defaults 8 and 64.
  * MDT_XIN_AUTO (default): the layout whose F-engine writes and X-engine
  * reads were fastest: pairs for the FP16 split up to 256 inputs, blocks for
- * the FP16 split above 256 inputs and for FP8, rows for FP32. */
+ * the FP16 split above 256 inputs and for FP8, rows for FP32.  The
+ * block-scaled FP8 format (512 and 1024 channels) is always in blocks; the
+ * layout set here then applies when the precision changes. */
 #define MDT_XIN_AUTO (-1)
 #define MDT_XIN_PAIRS 0
 #define MDT_XIN_ROWS 1
@@ -259,10 +262,21 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
 /* The X input's sample format follows the plan's precision: FP32 (re, im) for
  * MDT_XPREC_FP32; for MDT_XPREC_FP16_SPLIT each 8-byte sample holds FP16
  * (hi_re, hi_im, lo_re, lo_im) of x*s, s the plan's power-of-two scale,
- * hi = fp16(x*s), lo = fp16(x*s - hi); for MDT_XPREC_FP8 each sample is 4
- * bytes, FP16 (re, im) of x*s, and the F-engine also sums each channel's power
- * (FP8 needs Nchannels in [512, 4096] and Ntaps 4 or 8).  Run the F-engine at
- * the precision the X-engine will use. */
+ * hi = fp16(x*s), lo = fp16(x*s - hi).  MDT_XPREC_FP8 (Nchannels in
+ * [512, 4096], Ntaps 4 or 8) has two formats by channel count:
+ *   512 and 1024 channels: each sample is 2 bytes, e4m3 (re, im) of x / 2^eb
+ *     (nearest-even), eb one exponent per (input, block of 16 spectra) shared
+ *     by all channels, the smallest with max |re|, |im| over the block <=
+ *     448 2^eb; the blocks layout of 32 spectra (64-byte runs) always; the
+ *     exponents as E8M0 bytes eb + 127, [ceil(Ns/16)][NN], follow the samples
+ *     in the same buffer (mdt_fx_xinput_scales) and are the MFMA's scale
+ *     operands.  Every sample keeps e4m3's 3-bit precision whatever its
+ *     block's power, bursts included; results are the same bits on every run.
+ *   2048 and 4096 channels (an F-engine item spans fewer than 16 spectra):
+ *     each sample is 4 bytes, FP16 (re, im) of x*s, and the F-engine also
+ *     sums each channel's power (in a fixed order); the X-engine scales each
+ *     channel by a power of two to RMS ~ 32 and quantises on its way in.
+ * Run the F-engine at the precision the X-engine will use. */
 int mdt_fx_fengine(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative);
 /* The same F-engine on the reference's float F input, d_FInput [Ns+T-1][NN][N]
  * complex64 as decoded from the wire (MDT_OneStation_client.cu:898-907, 1022,
@@ -284,9 +298,14 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
                    double *Acc64);
 int mdt_fx_dump(mdt_fx_plan plan, const int8_t *InSignal, mdt_complex *XInputNative,
                 mdt_complex *Vis, double *Acc64);
-/* FP8 plans: the per-channel exponents e_c the X-engine applies (device int
- * array of Nchannels): samples are quantised as e4m3(x*s*2^e_c), RMS ~ 32. */
+/* FP8 plans at 2048 and 4096 channels: the per-channel exponents e_c the
+ * X-engine applies (device int array of Nchannels): samples are quantised as
+ * e4m3(x*s*2^e_c), RMS ~ 32. */
 int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents);
+/* FP8 plans at 512 and 1024 channels: where the block exponent bytes sit in
+ * the X input (byte offset) and how many blocks of 16 spectra there are (the
+ * bytes are [nblocks][NpolsxNelements]). */
+int mdt_fx_xinput_scales(mdt_fx_plan plan, size_t *offset, size_t *nblocks);
 /* the power-of-two scale s of the FP16 sample formats: the plan's, or with
  * gains the one the last F-engine launch used (then this synchronises the
  * plan's stream) */

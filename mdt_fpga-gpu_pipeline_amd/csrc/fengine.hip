@@ -200,7 +200,7 @@ struct PCfg {
   static constexpr int Q = QMAX < 1 ? 1 : (QMAX < 8 ? QMAX : 8);
   static constexpr int THREADS = Q * TPB;
   static constexpr int SPAN = SB * Q;                 // spectra per item
-  static constexpr int SPC = SPLIT == 2 ? 4 : 2;      // samples per 16-byte chunk of the output
+  static constexpr int SPC = SPLIT == 2 ? 4 : (SPLIT == 3 ? 8 : 2);  // samples per 16-byte output chunk
   static constexpr int CPR = SPAN / SPC;              // chunks per row
   // Staging rows (float2).  Rows of 8 spectra of 8 bytes (N = 4096): no pad,
   // each 16-byte chunk stored at chunk ^ ((row >> 1) & 3), so that both the
@@ -240,6 +240,7 @@ __global__ void arrange_taps_kernel(int N, int T, const float *__restrict__ W, f
 }
 
 bool persistent_shape(int N, int T) { return N >= 512 && N <= 4096 && (T == 4 || T == 8); }
+bool fp8_block_shape(int N) { return N == 512 || N == 1024; }
 
 int arrange_taps(int N, int T, const float *W, float *Wt, hipStream_t st) {
   if (!persistent_shape(N, T)) return fail(MDT_ERR_UNSUPPORTED, "no persistent F-engine for N=%d T=%d", N, T);
@@ -288,7 +289,7 @@ constexpr int fe_waves_per_simd() {
 //         against 9.86 ms, and with lanes 10.50 against 9.80 ms).  Taken
 //         together with lanes.
 template <int SPLIT>
-struct FeTuning {
+struct FeTuning {  // split 3 (e4m3) follows split 2's choices
   static constexpr bool pin = SPLIT == 1;
   static constexpr bool lanes = SPLIT != 0;
   static constexpr bool byte1 = SPLIT == 1;
@@ -303,6 +304,17 @@ __device__ __forceinline__ void store16(void *dst, unsigned w0, unsigned w1, uns
     __builtin_nontemporal_store(u32x4{w0, w1, w2, w3}, (u32x4 *)dst);
   else
     *(u32x4 *)dst = u32x4{w0, w1, w2, w3};
+}
+
+// two complex samples (a, b) -> e4m3 bytes (a.re, a.im, b.re, b.im) of x / scale
+// (the scaled conversion divides by its scale operand; |x / scale| <= 448 by
+// the block exponent, so nothing saturates), nearest-even
+typedef short fe_short2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned e4m3x2(float2 a, float2 b, float scale) {
+  fe_short2 r = {0, 0};
+  r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(r, a.x, a.y, scale, false);
+  r = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(r, b.x, b.y, scale, true);
+  return __builtin_bit_cast(unsigned, r);
 }
 
 // X input layout with its block and row pitch as constants: XL_ROWS, XL_PAIRS
@@ -320,6 +332,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   constexpr int NPASS = L + (RL > 1 ? 1 : 0);
   __shared__ float2 twl[C::TWN];
   __shared__ float2 buf[C::BUF];
+  __shared__ float wmax[SPLIT == 3 ? C::THREADS / 64 : 1];  // split 3: each wave's largest |re|, |im|
   const int tid = threadIdx.x;
   // the FP16 formats' scale: with gains it covers |g| x the bound (computed on
   // the device before the launch)
@@ -459,6 +472,23 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       }
     }
 
+    // split 3: the largest |re| or |im| of the spectra in the input (no
+    // spectrum past Ns), per wave into LDS; the item's blocks of 16 spectra
+    // take the maximum over their sub-blocks' waves after the next barrier
+    if constexpr (SPLIT == 3) {
+      float mx = 0.f;
+#pragma unroll
+      for (int sl = 0; sl < SB; sl++) {
+        const float wgt = sl < scount ? 1.f : 0.f;
+#pragma unroll
+        for (int m = 0; m < P; m++)
+          mx = fmaxf(mx, fmaxf(fabsf(acc[sl][m].x), fabsf(acc[sl][m].y)) * wgt);
+      }
+#pragma unroll
+      for (int msk = 32; msk >= 1; msk >>= 1) mx = fmaxf(mx, __shfl_xor(mx, msk, 64));
+      if ((tid & 63) == 0) wmax[tid >> 6] = mx;
+    }
+
     // corner turn through LDS, MR channel rounds per barrier pair: row
     // (channel) r of a round holds the item's SPAN spectra; each 16-byte
     // chunk of a row goes to one lane, the CPR chunks of a row to consecutive
@@ -478,7 +508,10 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     //   blocks: (blk*N + k)*cp + e*B + in_blk
     // with blk, in_blk the block of s0 + s1 and the place in it (s0, a
     // multiple of SPAN, and e are uniform).
-    constexpr int LGB = (SPLIT == 2 ? 4 : 3) + (XL == XL_BLOCKS ? 1 : 0), B = 1 << LGB;
+    // samples per block of the layout: 64 bytes of 8-byte or 4-byte samples in
+    // the pairs layout, 128 in the blocks layout; split 3 (blocks only) 64
+    // bytes of 2-byte samples, one K chunk of the FP8 X-engine
+    constexpr int LGB = SPLIT == 3 ? 5 : (SPLIT == 2 ? 4 : 3) + (XL == XL_BLOCKS ? 1 : 0), B = 1 << LGB;
     if constexpr (FeTuning<SPLIT>::lanes) {
       // = ibase + m0*rstride + v*vstride + xoff, xoff the lane's part,
       // computed once per item: with RV = THREADS / CPR rows per v, row =
@@ -505,6 +538,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       }
       const long rstride = (XL == XL_PAIRS ? (long)(TPB / 2) : (long)TPB) * a.xg.cp;  // per channel block of TPB
       const long vstride = (XL == XL_PAIRS ? (long)(RV / 2) : (long)RV) * a.xg.cp;    // per RV rows
+      float f8scale = 1.f;  // split 3: 2^eb of the lane's block of 16 spectra
 #pragma unroll
       for (int m0 = 0; m0 < P; m0 += MR) {
         __syncthreads();
@@ -516,12 +550,34 @@ fengine_persistent_kernel(FArgs a, long nitems) {
             buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
                 (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], split_sc) : acc[sl][m0 + mr];
         __syncthreads();
+        if constexpr (SPLIT == 3) {
+          if (m0 == 0) {  // the waves' maxima are visible: each block's exponent
+            constexpr int WPS = TPB / 64;  // waves per sub-block
+            auto block_max = [&](int b) {
+              float bm = 0.f;
+              for (int w = 4 * b * WPS; w < min(C::Q, 4 * b + 4) * WPS; w++) bm = fmaxf(bm, wmax[w]);
+              return bm;
+            };
+            const int eb = fp8_block_exponent(block_max(s1 >> 4));
+            f8scale = __builtin_ldexpf(1.0f, eb);
+            if (tid < SPAN / 16 && (s0 >> 4) + tid < a.nblk16)
+              a.scales[(long)((s0 >> 4) + tid) * a.NN + e] =
+                  (unsigned char)(fp8_block_exponent(block_max(tid)) + 127);
+          }
+        }
         const long rbase = ibase + (long)m0 * rstride + xoff;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           const long o = rbase + v * vstride;
           const float2 *src_ = src0 + v * RV * ROWW;
-          if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
+          if constexpr (SPLIT == 3) {  // 2-byte e4m3 samples, 8 per chunk, spectra past Ns zero
+            float2 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = s1 + i < scnt_item ? src_[i] : make_float2(0.f, 0.f);
+            store16<FeTuning<SPLIT>::stream>((char *)a.out + 2 * o, e4m3x2(v[0], v[1], f8scale),
+                                             e4m3x2(v[2], v[3], f8scale), e4m3x2(v[4], v[5], f8scale),
+                                             e4m3x2(v[6], v[7], f8scale));
+          } else if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
             unsigned *d4 = (unsigned *)a.out + o;
             const unsigned h0 = half_sample(src_[0], split_sc), h1 = half_sample(src_[1], split_sc);
             const unsigned h2 = half_sample(src_[2], split_sc), h3 = half_sample(src_[3], split_sc);
@@ -606,9 +662,28 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     __syncthreads();  // the staging buffer is the next item's exchange buffers
   }
   if constexpr (SPLIT == 2) {
+    // the workgroup's channel powers: its sub-blocks' sums added in the
+    // order q = 0..Q-1 through LDS, one row of the partial table per
+    // workgroup (its items are fixed by its index), reduced in order by
+    // power_reduce_kernel: the same bits on every run
+    float *red = (float *)buf;  // Q x N floats (the exchange buffers hold 2 Q (N + N/16))
 #pragma unroll
-    for (int m = 0; m < P; m++) atomicAdd(&a.power[t + m * TPB], pw[m]);
+    for (int m = 0; m < P; m++) red[q * N + t + m * TPB] = pw[m];
+    __syncthreads();
+    for (int k = tid; k < N; k += C::THREADS) {
+      float sum = 0.f;
+      for (int qq = 0; qq < C::Q; qq++) sum += red[qq * N + k];
+      a.power_part[(long)blockIdx.x * N + k] = sum;
+    }
   }
+}
+
+__global__ void power_reduce_kernel(int N, int rows, const float *__restrict__ part, float *__restrict__ power) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  float sum = 0.f;
+  for (int r = 0; r < rows; r++) sum += part[(long)r * N + k];
+  power[k] = sum;
 }
 
 template <int N, int T, int SPLIT, bool GAIN, int XL>
@@ -635,15 +710,28 @@ static int launch_persistent_k(const FArgs &a, hipStream_t st) {
   }
   long grid = full;
   if (grid > nitems) grid = nitems;
+  if (SPLIT == 2 && grid > a.power_slots) grid = a.power_slots;
   hipLaunchKernelGGL((fengine_persistent_kernel<N, T, SPLIT, GAIN, XL>), dim3((unsigned)grid), dim3(C::THREADS), 0,
                      st, a, nitems);
-  return check_launch("fengine_persistent_kernel");
+  MDT_TRY(check_launch("fengine_persistent_kernel"));
+  if constexpr (SPLIT == 2) {
+    hipLaunchKernelGGL(power_reduce_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, (int)grid, a.power_part,
+                       a.power);
+    MDT_TRY(check_launch("power_reduce_kernel"));
+  }
+  return MDT_SUCCESS;
 }
 
 template <int N, int T, int SPLIT>
 static int launch_persistent_s(const FArgs &a, hipStream_t st) {
   const bool gain = a.gain != nullptr;
   const XGeom &g = a.xg;
+  if constexpr (SPLIT == 3) {  // e4m3: blocks of 32 two-byte samples only
+    if (g.lgG == 0 && g.lgB == 5 && g.rp == 32 && g.bp == (long)N * g.cp)
+      return gain ? launch_persistent_k<N, T, 3, true, XL_BLOCKS>(a, st)
+                  : launch_persistent_k<N, T, 3, false, XL_BLOCKS>(a, st);
+    return fail(MDT_ERR_INVALID_VALUE, "the FP8 block-scaled X input is in blocks of 32 spectra");
+  }
   constexpr int LGB = SPLIT == 2 ? 4 : 3;  // pairs; blocks one more
   if (g.lgG == 1 && g.lgB == LGB && g.rp == (2L << LGB) && g.bp == (long)(N / 2) * g.cp)
     return gain ? launch_persistent_k<N, T, SPLIT, true, XL_PAIRS>(a, st)
@@ -663,9 +751,15 @@ static int launch_persistent(const FArgs &a, hipStream_t st) {
   switch (a.split) {
     case 0: return launch_persistent_s<N, T, 0>(a, st);
     case 1: return launch_persistent_s<N, T, 1>(a, st);
-    case 2: return launch_persistent_s<N, T, 2>(a, st);
-    default: return fail(MDT_ERR_INVALID_VALUE, "unknown F-engine output format %d", a.split);
+    case 2:
+      if constexpr (N >= 2048) return launch_persistent_s<N, T, 2>(a, st);
+      break;
+    case 3:
+      if constexpr (N <= 1024) return launch_persistent_s<N, T, 3>(a, st);
+      break;
+    default: break;
   }
+  return fail(MDT_ERR_INVALID_VALUE, "F-engine output format %d is not served at Nchannels=%d", a.split, N);
 }
 
 template <int N>
@@ -676,7 +770,7 @@ static int launch_fengine_n(const FArgs &a, hipStream_t st) {
     if (!fin && a.T == 4) return launch_persistent<N, 4>(a, st);
     if (!fin && a.T == 8) return launch_persistent<N, 8>(a, st);
   }
-  if (a.split == 2)
+  if (a.split >= 2)
     return fail(MDT_ERR_UNSUPPORTED,
                 "FP8 X-engine input needs int8 samples, Nchannels in [512, 4096] and Ntaps 4 or 8");
   const long nsb = (a.Ns + C::SB - 1) / C::SB;

@@ -17,11 +17,28 @@ struct FArgs {
   const float2 *gain;   // optional [NN] complex gains
   const float *delay;   // optional [NN] delays in samples
   int split;            // 0: FP32 (re, im); 1: FP16 (hi_re, hi_im, lo_re, lo_im) of x*split_scale;
-                        // 2: FP16 (re, im) of x*split_scale in 4-byte elements + channel power
+                        // 2: FP16 (re, im) of x*split_scale in 4-byte elements + channel power;
+                        // 3: e4m3 (re, im) in 2-byte elements + block exponents (scales)
   float split_scale;    // exact power of two (without gains)
   const float *dscale;  // with gains: the power-of-two scale for |g| x the bound, on the device
-  float *power;         // split 2: per-channel sum of |x*split_scale|^2 (zeroed by the caller)
+  float *power;         // split 2: per-channel sum of |x*split_scale|^2 [N]
+  float *power_part;    // split 2: per-workgroup partial sums [power_slots][N], added in order into power
+  int power_slots;      // split 2: rows of power_part (the persistent grid is capped to it)
+  unsigned char *scales;  // split 3: E8M0 exponent byte per (16-spectrum block, input), [nblk16][NN]
+  int nblk16;             // split 3: ceil(Ns / 16)
 };
+
+// split 3 (FP8 e4m3 with block scales): the exponent eb of a block whose
+// largest |re| or |im| is maxabs: the smallest integer with maxabs <= 448 2^eb
+// (448 = 0.875 2^9 is e4m3's largest finite value), clamped to E8M0's
+// [-127, 127]; samples are stored as e4m3(x / 2^eb), the byte eb + 127.
+__host__ __device__ inline int fp8_block_exponent(float maxabs) {
+  if (!(maxabs > 0.f)) return -127;
+  int E = 0;
+  const float m = frexpf(maxabs, &E);  // maxabs = m 2^E, m in [0.5, 1)
+  const int eb = E - (m <= 0.875f ? 9 : 8);
+  return eb < -127 ? -127 : (eb > 127 ? 127 : eb);
+}
 
 // The FP16 sample formats store x*s, s the largest power of two with
 // s*bound <= 2^15 (below the FP16 maximum 65504), bound a rigorous bound of
@@ -57,6 +74,9 @@ __device__ __forceinline__ float2 split_sample(float2 v, float sc) {
 int launch_fengine(const FArgs &a, int N, hipStream_t st);
 // shapes served by the persistent kernel (N in [512, 4096], T in {4, 8})
 bool persistent_shape(int N, int T);
+// FP8 sample formats by shape: e4m3 with block scales where an F-engine item
+// spans at least 16 spectra (N = 512, 1024), per-channel FP16 + power above
+bool fp8_block_shape(int N);
 // the window [T][N] re-arranged for the persistent kernel: Wt[((m*(T/4) + i/4)*(N/8) + t)*4 + i%4]
 // = W[i*N + t + m*N/8]
 int arrange_taps(int N, int T, const float *W, float *Wt, hipStream_t st);

@@ -24,7 +24,9 @@ struct mdt_fx_plan_s {
   double wbound = 0.0;     // rigorous bound of |X| without gains
   float *dscale = nullptr; // device: the scale of the last F-engine launch (f16_scale, or with gains)
   bool dscale_base = true; // dscale holds f16_scale
-  float *power = nullptr;  // FP8 path: per-channel power from the F-engine
+  float *power = nullptr;  // FP8 path, per-channel format: per-channel power from the F-engine
+  float *power_part = nullptr;  // its per-workgroup partial sums [power_slots][N]
+  int power_slots = 0;
   float *Wt = nullptr;     // window arranged for the persistent F-engine (when it serves the shape)
   int layout = MDT_XIN_AUTO;
   int row_pad = 8, chan_pad = 64;  // rows layout: padding (samples) after each row / channel
@@ -38,12 +40,17 @@ struct mdt_fx_plan_s {
     const long c = (long)NN * rp() + chan_pad;
     return xprec == MDT_XPREC_FP8 ? (c + 3) & ~3L : c;
   }
+  // FP8 sample formats: e4m3 with block exponents (512 and 1024 channels),
+  // FP16 with per-channel power (2048 and 4096)
+  bool f8block() const { return xprec == MDT_XPREC_FP8 && mdt::fp8_block_shape(N); }
   // the layout in use: MDT_XIN_AUTO picks, per X-engine, the one it reads
   // best with the F-engine's writes (measured): pairs only where each
   // channel's samples are read once (FP16 split, NN <= 256: one tile or the
   // whole-channel kernel); the tiled X-engines that re-read a channel do
-  // not reuse the partner channel's half of each 128-byte line
+  // not reuse the partner channel's half of each 128-byte line.  The
+  // block-scaled FP8 format has one layout, blocks of 32 spectra.
   int xin_layout() const {
+    if (f8block()) return MDT_XIN_BLOCKS;
     if (layout != MDT_XIN_AUTO) return layout;
     if (xprec == MDT_XPREC_FP32) return MDT_XIN_ROWS;
     if (xprec == MDT_XPREC_FP16_SPLIT && NN <= 256) return MDT_XIN_PAIRS;
@@ -54,6 +61,13 @@ struct mdt_fx_plan_s {
     mdt::XGeom g;
     const int lay = xin_layout();
     const int lg_esize = xprec == MDT_XPREC_FP8 ? 2 : 3;
+    if (f8block()) {  // 32 spectra of 2-byte e4m3 samples: 64-byte rows, one X-engine K chunk
+      g.lgB = 5;
+      g.rp = 32;
+      g.cp = (long)NN * g.rp;
+      g.bp = (long)N * g.cp;
+      return g;
+    }
     if (lay == MDT_XIN_ROWS) {
       g.rp = rp();
       g.cp = cp();
@@ -71,6 +85,9 @@ struct mdt_fx_plan_s {
     }
     return g;
   }
+  // block-scaled FP8: the exponent bytes [nblk16][NN] after the samples
+  int nblk16() const { return (Ns + 15) / 16; }
+  size_t scales_offset() const { return ((size_t)xin_samples() * 2 + 255) & ~(size_t)255; }
   // X input extent in samples
   long xin_samples() const {
     const mdt::XGeom g = geom();
@@ -125,7 +142,7 @@ __global__ void zero_kernel(float *p, int n) {
   if (i < n) p[i] = 0.f;
 }
 
-static size_t elem_bytes(const mdt_fx_plan_s *p) { return p->xprec == MDT_XPREC_FP8 ? 4 : 8; }
+static size_t elem_bytes(const mdt_fx_plan_s *p) { return p->f8block() ? 2 : (p->xprec == MDT_XPREC_FP8 ? 4 : 8); }
 
 static int check_xprec(int xprec, int N, int T) {
   if (xprec != MDT_XPREC_FP32 && xprec != MDT_XPREC_FP16_SPLIT && xprec != MDT_XPREC_FP8)
@@ -185,6 +202,18 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
     else
       st = arrange_taps(Nchannels, Ntaps, p->W, p->Wt, 0);
   }
+  if (st == MDT_SUCCESS && persistent_shape(Nchannels, Ntaps) && !fp8_block_shape(Nchannels)) {
+    // the per-channel FP8 format's partial power table: one row per
+    // resident workgroup of the persistent F-engine (at most 4 per CU)
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      st = fail(MDT_ERR_DEVICE, "device attributes");
+    p->power_slots = 4 * (cus > 0 ? cus : 256);
+    if (st == MDT_SUCCESS &&
+        hipMalloc(&p->power_part, sizeof(float) * (size_t)p->power_slots * Nchannels) != hipSuccess)
+      st = fail(MDT_ERR_ALLOC, "partial channel powers");
+  }
   if (st == MDT_SUCCESS) st = make_twiddles(Nchannels, &p->tw);
   if (st == MDT_SUCCESS && hipStreamSynchronize(0) != hipSuccess)
     st = fail(MDT_ERR_DEVICE, "plan table build");
@@ -193,6 +222,7 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
     (void)hipFree(p->W);
     (void)hipFree(p->power);
     (void)hipFree(p->dscale);
+    if (p->power_part) (void)hipFree(p->power_part);
     if (p->Wt) (void)hipFree(p->Wt);
     if (p->tw) (void)hipFree(p->tw);
     delete p;
@@ -226,7 +256,8 @@ int mdt_fx_set_gains(mdt_fx_plan plan, const mdt_complex *gains, const float *de
 int mdt_fx_xinput_bytes(mdt_fx_plan plan, size_t *bytes) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(bytes);
-  *bytes = elem_bytes(plan) * (size_t)plan->xin_samples();
+  *bytes = plan->f8block() ? plan->scales_offset() + (size_t)plan->nblk16() * plan->NN
+                           : elem_bytes(plan) * (size_t)plan->xin_samples();
   return MDT_SUCCESS;
 }
 
@@ -309,16 +340,18 @@ static int fengine(mdt_fx_plan plan, const int8_t *in8, const mdt_complex *fin, 
   a.split_scale = plan->f16_scale;
   a.dscale = plan->dscale;
   a.power = plan->power;
-  if (a.split != 0 && (plan->gain != nullptr || !plan->dscale_base)) {
+  a.power_part = plan->power_part;
+  a.power_slots = plan->power_slots;
+  if (plan->f8block()) {
+    a.split = 3;
+    a.scales = (unsigned char *)XInputNative + plan->scales_offset();
+    a.nblk16 = plan->nblk16();
+  }
+  if ((a.split == 1 || a.split == 2) && (plan->gain != nullptr || !plan->dscale_base)) {
     hipLaunchKernelGGL(gain_scale_kernel, dim3(1), dim3(256), 0, plan->stream, (const float2 *)plan->gain,
                        plan->NN, plan->wbound, plan->dscale);
     MDT_TRY(check_launch("gain_scale_kernel"));
     plan->dscale_base = plan->gain == nullptr;
-  }
-  if (a.split == 2) {  // graph-safe zeroing (a kernel, not a memset node)
-    hipLaunchKernelGGL(zero_kernel, dim3((plan->N + 255) / 256), dim3(256), 0, plan->stream, plan->power,
-                       plan->N);
-    MDT_TRY(check_launch("zero_kernel"));
   }
   return launch_fengine(a, plan->N, plan->stream);
 }
@@ -364,6 +397,11 @@ int mdt_fx_xengine(mdt_fx_plan plan, const mdt_complex *XInputNative, mdt_comple
   g.in_scale = plan->f16_scale;
   g.scale_dev = plan->dscale;  // as the last F-engine launch used it
   if (plan->xprec == MDT_XPREC_FP16_SPLIT) return xeng_launch_f16split(g, plan->stream);
+  if (plan->f8block()) {
+    g.f8scales = (const unsigned char *)XInputNative + plan->scales_offset();
+    g.nblk16 = plan->nblk16();
+    return xeng_launch_f8_block(g, plan->stream);
+  }
   if (plan->xprec == MDT_XPREC_FP8) return xeng_launch_f8(g, plan->power, plan->stream);
   return xeng_launch(g, XL_MK, XO_TRI, plan->stream);
 }
@@ -378,8 +416,22 @@ int mdt_fx_get_channel_exponents(mdt_fx_plan plan, int *exponents) {
   MDT_REQUIRE_PTR(plan);
   MDT_REQUIRE_PTR(exponents);
   if (plan->xprec != MDT_XPREC_FP8) return fail(MDT_ERR_INVALID_VALUE, "plan precision is not FP8");
+  if (plan->f8block())
+    return fail(MDT_ERR_INVALID_VALUE, "FP8 at %d channels scales per block of spectra (mdt_fx_xinput_scales)",
+                plan->N);
   return launch_chan_exponents(plan->N, plan->power, (float)plan->NN * (float)plan->Ns, exponents,
                                plan->stream);
+}
+
+int mdt_fx_xinput_scales(mdt_fx_plan plan, size_t *offset, size_t *nblocks) {
+  MDT_REQUIRE_PTR(plan);
+  MDT_REQUIRE_PTR(offset);
+  MDT_REQUIRE_PTR(nblocks);
+  if (!plan->f8block())
+    return fail(MDT_ERR_INVALID_VALUE, "the X input has no block exponents (FP8 at 512 or 1024 channels only)");
+  *offset = plan->scales_offset();
+  *nblocks = (size_t)plan->nblk16();
+  return MDT_SUCCESS;
 }
 
 int mdt_fx_plan_shape(mdt_fx_plan plan, int *NpolsxNelements, int *Nchannels, int *Ntaps, int *Nspectra,
@@ -410,6 +462,7 @@ int mdt_fx_plan_destroy(mdt_fx_plan plan) {
   if (plan->W) (void)hipFree(plan->W);
   if (plan->power) (void)hipFree(plan->power);
   if (plan->dscale) (void)hipFree(plan->dscale);
+  if (plan->power_part) (void)hipFree(plan->power_part);
   if (plan->Wt) (void)hipFree(plan->Wt);
   if (plan->tw) (void)hipFree(plan->tw);
   delete plan;

@@ -32,6 +32,9 @@ struct XArgs {
   int prec;       // MDT_XPREC_*
   float in_scale; // FP16 split: exact power-of-two pre-scale of the FP32 samples
   const float *scale_dev = nullptr;  // the same scale on the device, as the F-engine used it (nullable)
+  // FP8 block-scaled input: E8M0 exponent byte per (16-spectrum block, input), [nblk16][m]
+  const unsigned char *f8scales = nullptr;
+  int nblk16 = 0;
   // native X input geometry (XGeom, common.hpp) beyond the row pitch: batch
   // b's operand starts at base + (b >> xlgG) * Abatch + (b mod 2^xlgG) *
   // 2^xlgB, sample (row, s) at row * ld + (s >> xlgB) * xbp + (s mod
@@ -96,6 +99,7 @@ __device__ __forceinline__ void tri_block_out(float2 *__restrict__ C, double2 *_
 int xeng_launch(const XArgs &g, int layout, int out, hipStream_t st);
 int xeng_launch_f16split(const XArgs &g, hipStream_t st);
 int xeng_launch_f8(const XArgs &g, const float *power, hipStream_t st);
+int xeng_launch_f8_block(const XArgs &g, hipStream_t st);
 int launch_chan_exponents(int N, const float *power, float count, int *e, hipStream_t st);
 
 }  // namespace mdt

@@ -212,6 +212,170 @@ __global__ __launch_bounds__(256, 2) void xeng_f8_kernel(XArgs g, const float *p
     }
 }
 
+// ------------------------------------------------------------------ block-scaled e4m3 input
+// The F-engine writes e4m3 (re, im) samples x / 2^eb, eb one exponent per
+// (input, block of 16 spectra) shared by all channels (E8M0 bytes eb + 127,
+// [block][input]), in blocks of 32 spectra: a tile row's K chunk is one
+// 64-byte run and the 128 rows of a tile are 8 KB in a row.  No conversion on
+// the way into LDS; lane l of an MFMA holds 16 spectra (32 bytes) of row
+// l mod 32, exactly one block, so the block's exponent is the lane's scale
+// operand: D += sum (qa 2^ea)(qb 2^eb), the samples themselves.
+struct PrefetchB {
+  uint4 v[2];  // rows rq and rq + 64 of the tile: 8 samples (16 bytes) each
+  int nv[2];   // valid samples of each (0 for rows past NN, fewer in the last chunk)
+};
+
+__device__ __forceinline__ int f8_scale(const XArgs &g, int row, int blk) {
+  return (row < g.m && blk < g.nblk16) ? (int)g.f8scales[(long)blk * g.m + row] : 127;
+}
+
+// one operand's 128 rows x 32 spectra (64 bytes a row, rows 64 bytes apart)
+__device__ __forceinline__ void fetch8b(PrefetchB &p, const char *__restrict__ base, const XArgs &g, int row0,
+                                        int s0) {
+  const int seg = threadIdx.x & 3, rq = threadIdx.x >> 2;  // 4 lanes x 16 bytes per row
+  const long so = ((long)(s0 >> 5) * g.xbp + seg * 8) * 2;  // s0: a multiple of 32
+  const int nvs = min(8, max(0, g.k - s0 - seg * 8));
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int row = row0 + rq + 64 * h;
+    const bool ok = row < g.m;
+    p.v[h] = *(const uint4 *)(base + (long)(ok ? row : 0) * g.ldI * 2 + so);
+    p.nv[h] = ok ? nvs : 0;
+  }
+}
+
+// the lane's scale bytes for a chunk: I rows t = 0, 1, then J rows u = 0, 1
+__device__ __forceinline__ void fetch_scales(int (&sc)[4], const XArgs &g, int rowI, int rowJ, int s0, int kh) {
+  const int blk = (s0 >> 4) + kh;
+  sc[0] = f8_scale(g, rowI, blk);
+  sc[1] = f8_scale(g, rowI + 32, blk);
+  sc[2] = f8_scale(g, rowJ, blk);
+  sc[3] = f8_scale(g, rowJ + 32, blk);
+}
+
+// 16 bytes of e4m3 pairs into LDS, samples past nv zeroed
+__device__ __forceinline__ void stage8b(const PrefetchB &p, char *lds) {
+  const int seg = threadIdx.x & 3, rq = threadIdx.x >> 2;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    unsigned w[4] = {p.v[h].x, p.v[h].y, p.v[h].z, p.v[h].w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const unsigned lo = 2 * i < p.nv[h] ? 0x0000ffffu : 0u, hi = 2 * i + 1 < p.nv[h] ? 0xffff0000u : 0u;
+      w[i] &= lo | hi;
+    }
+    *(uint4 *)(lds + (rq + 64 * h) * Q8ROWB + seg * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void xeng_f8b_kernel(XArgs g) {
+  __shared__ __attribute__((aligned(16))) char lds[2 * Q8PLANE];  // [op I|J][row][k]
+
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+  const int batch = (int)(logical / g.tiles);
+  const int tile = (int)(logical % g.tiles);
+  int P = 0;
+  while ((P + 1) * (P + 2) / 2 <= tile) P++;
+  const int Q = tile - P * (P + 1) / 2;
+  const int bi = P, bj = Q;
+  const bool same = (bi == bj);
+  const char *base = (const char *)g.Abase + xin_batch(g, batch) * 2;  // 2-byte samples
+  const int NN = g.m, K = g.k;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int l32 = lane & 31, kh = lane >> 5;
+  bool act[2][2];
+  bool any = false;
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int ra = bi * 4 + wi * 2 + t, rb = bj * 4 + wj * 2 + u;
+      act[t][u] = (ra >= rb) && (ra * 32 < NN) && (rb * 32 < NN);
+      any |= act[t][u];
+    }
+
+  f32x16 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        accR[t][u][r] = 0.f;
+        accI[t][u][r] = 0.f;
+      }
+
+  char *ldsI = lds;
+  char *ldsJ = same ? lds : lds + Q8PLANE;
+  const int rowI = bi * Q8T + wi * 64 + l32, rowJ = bj * Q8T + wj * 64 + l32;  // the lane's MFMA rows (t, u = 0)
+  PrefetchB pI, pJ;
+  int scn[4];
+  fetch8b(pI, base, g, bi * Q8T, 0);
+  if (!same) fetch8b(pJ, base, g, bj * Q8T, 0);
+  fetch_scales(scn, g, rowI, rowJ, 0, kh);
+
+  for (int s0 = 0; s0 < K; s0 += Q8KC) {
+    __syncthreads();
+    stage8b(pI, lds);
+    if (!same) stage8b(pJ, lds + Q8PLANE);
+    const int scI[2] = {scn[0], scn[1]}, scJ[2] = {scn[2], scn[3]};
+    __syncthreads();
+    if (s0 + Q8KC < K) {
+      fetch8b(pI, base, g, bi * Q8T, s0 + Q8KC);
+      if (!same) fetch8b(pJ, base, g, bj * Q8T, s0 + Q8KC);
+      fetch_scales(scn, g, rowI, rowJ, s0 + Q8KC, kh);
+    }
+    if (any) {
+      const int koff = kh * 32;  // 32 bytes = 16 samples per half-wave: one block
+      i32x8 I[2], Pr[2], J[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        I[t] = ld_op(ldsI + (wi * 64 + t * 32 + l32) * Q8ROWB + koff);
+        Pr[t] = rot_conj8(I[t]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) J[u] = ld_op(ldsJ + (wj * 64 + u * 32 + l32) * Q8ROWB + koff);
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          if (!act[t][u]) continue;
+          accR[t][u] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(I[t], J[u], accR[t][u], 0, 0, 0, scI[t], 0,
+                                                                      scJ[u]);
+          accI[t][u] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(Pr[t], J[u], accI[t][u], 0, 0, 0, scI[t], 0,
+                                                                      scJ[u]);
+        }
+    }
+  }
+  if (!any) return;
+
+  const float oscale = g.alpha.x;  // the samples are x itself
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (!act[t][u]) continue;
+      tri_block_out<8>(g.Cbase, (double2 *)g.acc64, (long)batch * g.Cbatch, NN, bi * Q8T + wi * 64 + t * 32, kh,
+                       bj * Q8T + wj * 64 + u * 32 + l32, oscale, accR[t][u], accI[t][u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+int xeng_launch_f8_block(const XArgs &g0, hipStream_t st) {
+  XArgs g = g0;
+  const int nb = (g.m + Q8T - 1) / Q8T;
+  g.tiles = nb * (nb + 1) / 2;
+  const long total = (long)g.tiles * g.batch;
+  MDT_TRY(check_grid(total, 256, "X-engine"));
+  if (g.f8scales == nullptr) return fail(MDT_ERR_INVALID_VALUE, "FP8 block-scaled input needs its scales");
+  hipLaunchKernelGGL(xeng_f8b_kernel, dim3((unsigned)total), dim3(256), 0, st, g);
+  return check_launch("xeng_f8b_kernel");
+}
+
 int xeng_launch_f8(const XArgs &g0, const float *power, hipStream_t st) {
   XArgs g = g0;
   const int nb = (g.m + Q8T - 1) / Q8T;

@@ -115,6 +115,7 @@ _SIGS = {
     "mdt_fx_dump": [_VP, _VP, _VP, _VP, _VP],
     "mdt_fx_plan_destroy": [_VP],
     "mdt_fx_get_channel_exponents": [_VP, _VP],
+    "mdt_fx_xinput_scales": [_VP, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)],
     "mdt_fx_get_split_scale": [_VP, ctypes.POINTER(_F)],
     "mdt_fx_plan_shape": [_VP, _VP, _VP, _VP, _VP, _VP],
     "mdt_stream_create": [ctypes.POINTER(_VP), _VP],
@@ -447,21 +448,39 @@ class FxPlan:
         check(lib().mdt_fx_xinput_bytes(self._h, ctypes.byref(nbytes)))
         return nbytes.value
 
+    @property
+    def f8block(self):
+        """FP8 with block exponents (512 and 1024 channels): 2-byte e4m3
+        samples, the exponents after them in the X input"""
+        return self.xprec == XPREC_FP8 and self.N in (512, 1024)
+
     def alloc_xinput(self):
         """X input buffer (flat, mdt_fx_xinput_bytes); xinput_view() gives
         the samples as [N][NN][Ns]."""
         nbytes = self.xinput_bytes()
+        if self.f8block:  # 2-byte e4m3 samples and the exponent bytes
+            return torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         if self.xprec == XPREC_FP8:  # 4-byte samples: FP16 (re, im)
             return torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
         return torch.empty(nbytes // 8, dtype=torch.complex64, device="cuda")
 
+    def xinput_scales(self, buf):
+        """block-scaled FP8 plans: the E8M0 exponent bytes (eb + 127) of the
+        blocks of 16 spectra, uint8 [ceil(Ns/16)][NN] (a view of buf)"""
+        off, nblk = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().mdt_fx_xinput_scales(self._h, ctypes.byref(off), ctypes.byref(nblk)))
+        b = buf.view(torch.uint8)
+        return b[off.value: off.value + nblk.value * self.NN].view(nblk.value, self.NN)
+
     def xinput_view(self, buf):
-        """The samples as [N][NN][Ns] complex64, or for FP8 plans float16
-        [N][NN][Ns][2] (re, im) of x*s: a view of buf in the rows layout, a
-        copy in the others."""
+        """The samples as [N][NN][Ns] complex64; for FP8 plans at 2048 and
+        4096 channels float16 [N][NN][Ns][2] (re, im) of x*s, at 512 and 1024
+        channels the e4m3 codes uint8 [N][NN][Ns][2] (re, im) of x / 2^eb
+        (xinput_scales); a view of buf in the rows layout, a copy in the
+        others."""
         B, bp, G, cp, rp = self.xinput_strides()
         f8 = self.xprec == XPREC_FP8
-        src = buf.view(torch.float16) if f8 else buf
+        src = buf.view(torch.uint8) if self.f8block else (buf.view(torch.float16) if f8 else buf)
         u = 2 if f8 else 1   # tensor elements per sample
         tail, tstr = ((2,), (1,)) if f8 else ((), ())
         if G == 1 and bp == 0:

@@ -4,7 +4,8 @@ a buffer filled through the formula the C header documents,
   (s // block) * block_pitch + (k // chan_group) * chan_pitch
   + e * row_pitch + (k % chan_group) * block + s % block,
 must come back as the [N][NN][Ns] samples (complex64, or float16 (re, im)
-pairs for the FP8 path).  No GPU: the strides are given, not queried."""
+pairs for the per-channel FP8 format, or e4m3 byte pairs for the
+block-scaled one).  No GPU: the strides are given, not queried."""
 import types
 
 import numpy as np
@@ -14,8 +15,8 @@ import torch
 import mdtfx
 
 
-def _fake_plan(N, NN, Ns, xprec, strides):
-    return types.SimpleNamespace(N=N, NN=NN, Ns=Ns, xprec=xprec, xinput_strides=lambda: strides)
+def _fake_plan(N, NN, Ns, xprec, strides, f8block=False):
+    return types.SimpleNamespace(N=N, NN=NN, Ns=Ns, xprec=xprec, xinput_strides=lambda: strides, f8block=f8block)
 
 
 def _layouts(N, NN, Ns, esize):
@@ -76,3 +77,22 @@ def test_rows_view_is_a_view():
     v[1, 2, 3] = 5 + 6j
     B, bp, G, cp, rp = strides
     assert buf[1 * cp + 2 * rp + 3] == 5 + 6j
+
+
+@pytest.mark.parametrize("N,NN,Ns", [(8, 3, 40), (4, 5, 70), (2, 1, 32)])
+def test_block_fp8_view(N, NN, Ns):
+    """the block-scaled FP8 format: 2-byte (re, im) e4m3 codes in blocks of
+    32 spectra (row pitch 32, channel pitch NN*32, block pitch N*NN*32)"""
+    strides = (32, N * NN * 32, 1, NN * 32, 32)
+    B, bp, G, cp, rp = strides
+    total = _size(N, Ns, strides)
+    k, e, s = np.meshgrid(np.arange(N), np.arange(NN), np.arange(Ns), indexing="ij")
+    off = (s // B) * bp + k * cp + e * rp + s % B
+    assert off.max() < total and len(np.unique(off)) == off.size
+    re, im = ((k * 7 + e * 3 + s) % 251).astype(np.uint8), ((k + e * 5 + s * 3) % 253).astype(np.uint8)
+    buf = np.zeros((total, 2), np.uint8)
+    buf[off, 0], buf[off, 1] = re, im
+    plan = _fake_plan(N, NN, Ns, mdtfx.XPREC_FP8, strides, f8block=True)
+    v = plan_view(plan, torch.from_numpy(buf.reshape(-1)))
+    assert tuple(v.shape) == (N, NN, Ns, 2) and v.dtype == torch.uint8
+    assert np.array_equal(v[..., 0].numpy(), re) and np.array_equal(v[..., 1].numpy(), im)
