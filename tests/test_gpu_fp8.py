@@ -71,10 +71,19 @@ BLOCK_CASES = [
     (512, 1024, 4, 64, 300, 40.0, 4.0, None),    # wide-band shape, 64 of its 8192 spectra
     (96, 512, 8, 100, 77, 40.0, 4.0, None),      # partial 128-row tile, 100 = 6 x 16 + 4 spectra
     (64, 1024, 4, 96, 100, 1.0, 1.0, (40, 58, 60.0)),   # a burst ~60x the noise in 18 frames
+    # a burst in frames 20-29: spectra 17-29, so the two 16-spectrum halves of
+    # the first 32-spectrum MFMA step get different exponents
+    (64, 1024, 4, 96, 100, 1.0, 1.0, (20, 30, 60.0)),
+    # whole-channel kernel: 300 inputs (a ragged 32-row block, all four
+    # workgroups of a channel), 70 spectra (the last 64-spectrum chunk half
+    # past the dump); 200 inputs (one triangle workgroup), 48 spectra
+    (300, 512, 4, 70, 100, 40.0, 4.0, None),
+    (200, 1024, 4, 48, 500, 40.0, 4.0, None),
 ]
 
 
-@pytest.fixture(scope="module", params=BLOCK_CASES, ids=lambda c: f"NN{c[0]}_N{c[1]}_T{c[2]}_Ns{c[3]}")
+@pytest.fixture(scope="module", params=BLOCK_CASES,
+                ids=lambda c: f"NN{c[0]}_N{c[1]}_T{c[2]}_Ns{c[3]}" + (f"_burst{c[7][0]}" if c[7] else ""))
 def blockcase(request):
     NN, N, T, Ns, kt, amp, sigma, burst = request.param
     print(f"fp8 block NN={NN} N={N} T={T} Ns={Ns} tone bin {kt} burst {burst}", flush=True)
