@@ -27,7 +27,8 @@ struct FCfg {
   static constexpr int TPB = N / P;
   static constexpr int EB = TPB >= 64 ? 1 : 64 / TPB;  // inputs per block
   static constexpr int THREADS = TPB * EB;
-  static constexpr int SB = N <= 4096 ? 8 : (N == 8192 ? 2 : 1);  // spectra per block
+  // spectra per block: 8 x 8 points, 4 x 16 points (N = 1024) in registers
+  static constexpr int SB = N <= 4096 ? (P == 16 ? 4 : 8) : (N == 8192 ? 2 : 1);
   static constexpr int FFT_LDS = lds_entries<N>();
   static constexpr int ROW = SB + 1;  // padded staging row (float2)
   static constexpr int STAGE_LDS = TPB * ROW;
@@ -175,8 +176,10 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   }
 }
 
-// Persistent kernel for N in [512, 4096] (8 points per thread, TPB = N/8
-// threads per transform) and T in {4, 8}.  A workgroup is Q sub-blocks of TPB
+// Persistent kernel for N in [512, 4096] (P = fft_points(N) points per thread:
+// 16 at N = 1024, so that a transform is one wave and its exchanges need no
+// workgroup barrier; 8 otherwise; TPB = N/P threads per transform) and T in
+// {4, 8}.  A workgroup is Q sub-blocks of TPB
 // threads; an item is one input and 4Q consecutive spectra, sub-block q
 // taking spectra 4q..4q+3.  Each sub-block runs the FIR and the four
 // transforms with its own exchange buffer (the twiddle table is shared), then
@@ -193,10 +196,11 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
 // mdt_pps_batch_i8 + mdt_fft_exec_c2c, so every path agrees bit for bit.
 template <int N, int SPLIT, bool GAIN>
 struct PCfg {
-  static constexpr int P = 8, TPB = N / P, SB = 4;
+  static constexpr int P = fft_points(N), TPB = N / P, SB = 4;
   // sub-blocks per workgroup: up to 1024 threads (512 with the gain stage,
-  // whose phasors need the registers of two waves per SIMD)
-  static constexpr int QMAX = (GAIN ? 512 : 1024) / TPB;
+  // whose phasors need the registers of two waves per SIMD, and with 16
+  // points per thread)
+  static constexpr int QMAX = ((GAIN || P == 16) ? 512 : 1024) / TPB;
   static constexpr int Q = QMAX < 1 ? 1 : (QMAX < 8 ? QMAX : 8);
   static constexpr int THREADS = Q * TPB;
   static constexpr int SPAN = SB * Q;                 // spectra per item
@@ -230,7 +234,7 @@ __device__ __forceinline__ int stage_idx(int row, int col) {
 __global__ void arrange_taps_kernel(int N, int T, const float *__restrict__ W, float *__restrict__ Wt) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= N * T) return;
-  const int TPB = N / 8, T4 = T / 4;
+  const int TPB = N / fft_points(N), T4 = T / 4;
   const int ii = idx & 3;
   int r = idx >> 2;
   const int t = r % TPB;
@@ -262,10 +266,11 @@ constexpr int kBufferWord3 = 0x00020000;
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
 // waves per SIMD the register budget targets: 4 (128 VGPRs) for the plain
-// outputs; the gain stage and the FP8 power sums at N >= 2048 need more room
+// outputs; the gain stage, the FP8 power sums at N >= 2048 and 16 points per
+// thread (N = 1024: 4 spectra x 16 points are 128 registers) need more room
 template <int N, int SPLIT, bool GAIN>
 constexpr int fe_waves_per_simd() {
-  return GAIN ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
+  return (GAIN || fft_points(N) == 16) ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
 }
 
 // Code-generation choices per output format, each the faster one measured
@@ -330,10 +335,20 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   constexpr int L = full_passes(N, P);
   constexpr int RL = N / ipow_c(P, L);
   constexpr int NPASS = L + (RL > 1 ? 1 : 0);
+  // the arranged taps in LDS where they fit (up to 32 KB: 1024 channels x 8
+  // taps, 2048 x 4): read with ds_read_b128, so the FIR's only vector-memory
+  // loads are its frames and each wait counts down through them in order;
+  // read through L1/L2 otherwise, where a tap load queued behind frame loads
+  // waits for all of them
+  constexpr bool TAPS_LDS = T * N * 4 <= 32768 && (C::TWN + C::BUF) * 8 + T * N * 4 <= 150 * 1024;
   __shared__ float2 twl[C::TWN];
   __shared__ float2 buf[C::BUF];
   __shared__ float wmax[SPLIT == 3 ? C::THREADS / 64 : 1];  // split 3: each wave's largest |re|, |im|
+  __shared__ v4u wtl[TAPS_LDS ? T * N / 4 : 1];
   const int tid = threadIdx.x;
+  if constexpr (TAPS_LDS) {
+    for (int idx = tid; idx < T * N / 4; idx += C::THREADS) wtl[idx] = ((const v4u *)a.Wt)[idx];
+  }
   // the FP16 formats' scale: with gains it covers |g| x the bound (computed on
   // the device before the launch)
   const float split_sc = GAIN ? *a.dscale : a.split_scale;
@@ -397,7 +412,11 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       float w[T];
 #pragma unroll
       for (int i4 = 0; i4 < T / 4; i4++) {
-        const v4u qv = __builtin_amdgcn_raw_buffer_load_b128(rtap, 16 * tl, 16 * ((m * (T / 4) + i4) * TPB), 0);
+        v4u qv;
+        if constexpr (TAPS_LDS)
+          qv = wtl[(m * (T / 4) + i4) * TPB + tl];
+        else
+          qv = __builtin_amdgcn_raw_buffer_load_b128(rtap, 16 * tl, 16 * ((m * (T / 4) + i4) * TPB), 0);
         w[4 * i4] = __uint_as_float(qv[0]);
         w[4 * i4 + 1] = __uint_as_float(qv[1]);
         w[4 * i4 + 2] = __uint_as_float(qv[2]);

@@ -29,8 +29,26 @@ constexpr int full_passes(int n, int p) {
   return l;
 }
 
-// points per thread for an N-point transform
-constexpr int fft_points(int N) { return N <= 4096 ? 8 : 16; }
+// points per thread for an N-point transform: 16 at N = 1024, so that one
+// transform is one wave (64 threads) and its LDS exchanges need no workgroup
+// barrier; 8 up to 4096 (a 512-point transform is one wave already); 16 above
+constexpr int fft_points(int N) { return N == 1024 ? 16 : (N <= 4096 ? 8 : 16); }
+
+// exchange synchronisation between passes: a transform held by one wave (or
+// part of one) only orders its own LDS accesses (a wave's LDS operations
+// execute in order; the wavefront-scope fence keeps the compiler from moving
+// one lane's accesses past another's); larger transforms span waves and
+// need the workgroup barrier
+template <int TPB>
+__device__ __forceinline__ void fft_exchange_sync() {
+  if constexpr (TPB <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
 
 // launch shape of the standalone N-point transforms: FPB transforms of TPB
 // threads (P points each) per workgroup
@@ -221,7 +239,7 @@ __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, co
   constexpr int NS = ipow_c(P, PASS);
   constexpr int TPB = N / P;
   if constexpr (PASS > 0) {
-    __syncthreads();
+    fft_exchange_sync<TPB>();
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (TPB % 16 == 0) {  // lpad(t + m*TPB) = lpad(t) + m*TPB*17/16
       const float2 *rp = lds + lpad(t);
@@ -231,7 +249,7 @@ __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, co
 #pragma unroll
       for (int m = 0; m < P; m++) x[m] = lds[lpad(t + m * TPB)];
     }
-    __syncthreads();
+    fft_exchange_sync<TPB>();
   }
   fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x, t, lds, tw);
   if constexpr (PASS + 1 < NPASS) fft_stage<N, P, INV, PASS + 1>(x, t, lds, tw);
@@ -249,12 +267,12 @@ __device__ __forceinline__ void fft_stage_multi(float2 (&x)[NX][P], int t, float
   constexpr int NS = ipow_c(P, PASS);
   constexpr int TPB = N / P;
   if constexpr (PASS > 0) {
-    __syncthreads();
+    fft_exchange_sync<TPB>();
 #pragma unroll
     for (int v = 0; v < NX; v++)
 #pragma unroll
       for (int m = 0; m < P; m++) x[v][m] = lds[v * lds_entries<N>() + lpad(t + m * TPB)];
-    __syncthreads();
+    fft_exchange_sync<TPB>();
   }
 #pragma unroll
   for (int v = 0; v < NX; v++)
@@ -269,8 +287,8 @@ __device__ __forceinline__ void fft_block_multi(float2 (&x)[NX][P], int t, float
 
 // Transform one N-point vector held as x[m] = element t + m*(N/P).  All
 // threads of the block must call this the same number of times (it contains
-// __syncthreads when N > P).  lds: lds_entries<N>() float2 private to this
-// transform.
+// __syncthreads when the transform spans more than one wave).  lds:
+// lds_entries<N>() float2 private to this transform.
 template <int N, int P, bool INV, typename TW>
 __device__ __forceinline__ void fft_block_tw(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
   static_assert(N % P == 0 && N >= P, "N must be a multiple of P");
