@@ -345,9 +345,17 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   __shared__ float2 buf[C::BUF];
   __shared__ float wmax[SPLIT == 3 ? C::THREADS / 64 : 1];  // split 3: each wave's largest |re|, |im|
   __shared__ v4u wtl[TAPS_LDS ? T * N / 4 : 1];
+  // split 2: the channel powers summed in LDS, one row per sub-block, each
+  // entry owned by one thread, items added in order (P more registers do not
+  // fit beside the transforms: a 1024-thread workgroup has 128 a thread)
+  constexpr bool PW_LDS = SPLIT == 2;
+  __shared__ float pwl[PW_LDS ? C::Q * N : 1];
   const int tid = threadIdx.x;
   if constexpr (TAPS_LDS) {
     for (int idx = tid; idx < T * N / 4; idx += C::THREADS) wtl[idx] = ((const v4u *)a.Wt)[idx];
+  }
+  if constexpr (PW_LDS) {
+    for (int idx = tid; idx < C::Q * N; idx += C::THREADS) pwl[idx] = 0.f;
   }
   // the FP16 formats' scale: with gains it covers |g| x the bound (computed on
   // the device before the launch)
@@ -379,9 +387,11 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   const __amdgpu_buffer_rsrc_t rtap =
       __builtin_amdgcn_make_buffer_rsrc((void *)a.Wt, (short)0, T * N * 4, kBufferWord3);
   float2 *fbuf = buf + q * C::FFT_LDS;  // this sub-block's exchange buffer
-  float pw[P];  // split 2: this thread's channels' power over all its items
-#pragma unroll
-  for (int m = 0; m < P; m++) pw[m] = 0.f;
+  // |x|^2 of this thread's channel t + m TPB added to its power sum
+  auto add_power = [&](int m, float xr, float xi) {
+    float &p = pwl[q * N + t + m * TPB];
+    p = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, p));
+  };
   for (long base = 0; base < nitems; base += G) {
     // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
     // spectrum blocks of one input: shared FIR frames, adjacent output pieces)
@@ -463,7 +473,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
 #pragma unroll
         for (int m = 0; m < P; m++) {
           const float xr = acc[sl][m].x * wgt, xi = acc[sl][m].y * wgt;
-          pw[m] = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, pw[m]));
+          add_power(m, xr, xi);
         }
       }
     }
@@ -486,7 +496,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
           for (int sl = 0; sl < SB; sl++) {
             const float wgt = sl < scount ? split_sc : 0.f;
             const float xr = acc[sl][m].x * wgt, xi = acc[sl][m].y * wgt;
-            pw[m] = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, pw[m]));
+            add_power(m, xr, xi);
           }
       }
     }
@@ -682,16 +692,13 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   }
   if constexpr (SPLIT == 2) {
     // the workgroup's channel powers: its sub-blocks' sums added in the
-    // order q = 0..Q-1 through LDS, one row of the partial table per
+    // order q = 0..Q-1 from LDS, one row of the partial table per
     // workgroup (its items are fixed by its index), reduced in order by
     // power_reduce_kernel: the same bits on every run
-    float *red = (float *)buf;  // Q x N floats (the exchange buffers hold 2 Q (N + N/16))
-#pragma unroll
-    for (int m = 0; m < P; m++) red[q * N + t + m * TPB] = pw[m];
     __syncthreads();
     for (int k = tid; k < N; k += C::THREADS) {
       float sum = 0.f;
-      for (int qq = 0; qq < C::Q; qq++) sum += red[qq * N + k];
+      for (int qq = 0; qq < C::Q; qq++) sum += pwl[qq * N + k];
       a.power_part[(long)blockIdx.x * N + k] = sum;
     }
   }

@@ -31,8 +31,10 @@ constexpr int full_passes(int n, int p) {
 
 // points per thread for an N-point transform: 16 at N = 1024, so that one
 // transform is one wave (64 threads) and its LDS exchanges need no workgroup
-// barrier; 8 up to 4096 (a 512-point transform is one wave already); 16 above
-constexpr int fft_points(int N) { return N == 1024 ? 16 : (N <= 4096 ? 8 : 16); }
+// barrier, and at N = 2048 (three passes, two exchanges instead of three);
+// 8 up to 4096 (a 512-point transform is one wave already; at 4096 the
+// F-engine's FP16 hi/lo output measured twice as slow with 16); 16 above
+constexpr int fft_points(int N) { return (N == 1024 || N == 2048) ? 16 : (N <= 4096 ? 8 : 16); }
 
 // exchange synchronisation between passes: a transform held by one wave (or
 // part of one) only orders its own LDS accesses (a wave's LDS operations
