@@ -12,14 +12,20 @@
  *   orc_pps_*         impulse -> window column, constant -> tap sum,
  *                     linearity, FP64 vs FP32 agreement
  *   orc_fft_* / dft   brute-force DFT at small N, numpy.fft (pocketfft)
- *                     golden vectors, Parseval, tones, shift theorem
+ *                     golden vectors (powers of two and 1000, 1536, 3000,
+ *                     4099), Parseval, tones, shift theorem
  *   orc_reorder       index formula, inverse permutation, raw bit patterns
  *   orc_xengine_*     Hermitian symmetry, Cauchy-Schwarz, identical inputs,
- *                     numpy einsum golden vectors
+ *                     numpy einsum golden vectors; the complex128 branch
+ *                     (orc_xengine_full_f64) against the same golden vectors
+ *                     and against the complex64 branch on identical values
  *   orc_unpack_u32    exhaustive 2^32 comparison of the literal reference
  *                     expression against the two's-complement closed form
  *   orc_depacketise   hand-built packets with known destinations
  *   orc_calib_*       FP64 central finite differences, closed-form cases
+ *   orc_fx_chain_*    numpy FP64 FIR -> numpy.fft -> einsum golden chain
+ * The test-side e4m3 quantiser (tests/e4m3.py) is pinned against torch's
+ * float8_e4m3fn conversion.
  */
 #include "mdt_oracle.h"
 
