@@ -273,6 +273,36 @@ constexpr int fe_waves_per_simd() {
   return (GAIN || fft_points(N) == 16) ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
 }
 
+// Frame prefetch (4096 channels x 8 taps, FP32 and FP16 hi/lo outputs): the
+// next item's 15 input frames (120 KB of int8 (I, Q)) are copied into LDS by
+// buffer-to-LDS DMA while this item is corner-turned and stored, and the FIR
+// reads its frames from LDS.  The FIR's 88 two-byte global loads per thread
+// and item had made it wait on memory latency several times per item (C3
+// F-engine 8.33 ms; with the frames prefetched 7.0 ms, same bits).  The
+// frame area doubles as three of the four exchange buffers of the two
+// transforms each sub-block advances together; the fourth holds the
+// corner-turn staging, so the DMA never targets LDS in use.  The last pass's
+// twiddles are held in registers (the LDS table keeps passes 1 and 2).
+template <int N, int T, int SPLIT, bool GAIN>
+constexpr bool fe_dma_mode() {
+  return N == 4096 && T == 8 && !GAIN && (SPLIT == 0 || SPLIT == 1);
+}
+
+// s_barrier after this wave's LDS accesses completed, without the release
+// fence of __syncthreads(): that fence also waits for every load in flight,
+// the frame DMA included (vmcnt(0)); the staging barriers only order LDS
+template <bool RAW>
+__device__ __forceinline__ void stage_barrier() {
+  if constexpr (RAW) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unchanged
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Code-generation choices per output format, each the faster one measured
 // (C3 F-engine: FP16 hi/lo and FP32 outputs at N = 4096; C4: the FP8 path's
 // output at N = 1024).  At the 128-VGPR budget the register allocation
@@ -330,7 +360,7 @@ template <int N, int T, int SPLIT, bool GAIN, int XL>
 __global__ __launch_bounds__((PCfg<N, SPLIT, GAIN>::THREADS), (fe_waves_per_simd<N, SPLIT, GAIN>())) void
 fengine_persistent_kernel(FArgs a, long nitems) {
   using C = PCfg<N, SPLIT, GAIN>;
-  constexpr int P = C::P, TPB = C::TPB, SB = C::SB, SPAN = C::SPAN, ROWW = C::ROWW, MR = C::MR;
+  constexpr int P = C::P, TPB = C::TPB, SB = C::SB, SPAN = C::SPAN, ROWW = C::ROWW;
   constexpr int SPC = C::SPC, CPR = C::CPR;
   constexpr int L = full_passes(N, P);
   constexpr int RL = N / ipow_c(P, L);
@@ -341,8 +371,17 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   // read through L1/L2 otherwise, where a tap load queued behind frame loads
   // waits for all of them
   constexpr bool TAPS_LDS = T * N * 4 <= 32768 && (C::TWN + C::BUF) * 8 + T * N * 4 <= 150 * 1024;
-  __shared__ float2 twl[C::TWN];
-  __shared__ float2 buf[C::BUF];
+  constexpr bool DMA = fe_dma_mode<N, T, SPLIT, GAIN>();
+  constexpr int NF = SPAN + T - 1;                       // input frames per item
+  constexpr int TWN = DMA ? pass_tw_offset<P>(L - 1) : C::TWN;
+  constexpr int MR = DMA ? 1 : C::MR;                    // staging rounds per barrier pair
+  static_assert(!DMA || (RL == 1 && NF * N / 4 >= 3 * C::FFT_LDS && C::FFT_LDS >= TPB * ROWW && C::Q == 2),
+                "frame area holds three exchange buffers, the fourth the staging");
+  __shared__ float2 twl[TWN];
+  __shared__ float2 buf[DMA ? 1 : C::BUF];
+  __shared__ float2 frb[DMA ? NF * N / 4 : 1];  // DMA: the item's frames, [f][channel] int8 (I, Q)
+  __shared__ float2 xb[DMA ? C::FFT_LDS : 1];   // DMA: fourth exchange buffer and the staging
+  float2 *const sbuf = DMA ? xb : buf;         // corner-turn staging
   __shared__ float wmax[SPLIT == 3 ? C::THREADS / 64 : 1];  // split 3: each wave's largest |re|, |im|
   __shared__ v4u wtl[TAPS_LDS ? T * N / 4 : 1];
   // split 2: the channel powers summed in LDS, one row per sub-block, each
@@ -367,7 +406,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
 
   // per-pass twiddle table: pass p >= 1 (stage NS = 8^p, radix R) holds
   // W_{NS*R}^{jm*r} at [offset(p) + r*NS + jm]
-  for (int idx = tid; idx < C::TWN; idx += C::THREADS) {
+  for (int idx = tid; idx < TWN; idx += C::THREADS) {
     int pass = 1, off = 0, ns = P;
     while (pass < NPASS - 1 && idx >= off + ns * P) {
       off += ns * P;
@@ -378,6 +417,11 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     const int rel = idx - off, r = rel / ns, jm = rel % ns;
     twl[idx] = a.tw[jm * r * (N / (ns * R))];
   }
+  LastRegTw<N, P> tw2{twl, {}};  // DMA mode: the last pass's twiddles in registers
+  if constexpr (DMA) {
+#pragma unroll
+    for (int r = 0; r < P; r++) tw2.w[r] = a.tw[t * r];
+  }
   __syncthreads();
 
   const int nsb = (a.Ns + SPAN - 1) / SPAN;
@@ -387,16 +431,41 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   const __amdgpu_buffer_rsrc_t rtap =
       __builtin_amdgcn_make_buffer_rsrc((void *)a.Wt, (short)0, T * N * 4, kBufferWord3);
   float2 *fbuf = buf + q * C::FFT_LDS;  // this sub-block's exchange buffer
+  // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
+  // spectrum blocks of one input: shared FIR frames, adjacent output pieces)
+  // go to the same XCD.
+  auto logical_of = [&](long base) { return (G % 8 == 0) ? base + (b % 8) * (G / 8) + b / 8 : base + b; };
+  // DMA mode: item lg's NF frames (clamped to the input's last frame) into
+  // frb, frame f at frb bytes [f N 2, (f + 1) N 2); each wave-instruction
+  // moves 1 KB (64 lanes x 16 bytes, written in lane order from the LDS base
+  // in M0), 8 per frame
+  auto dma_frames = [&](long lg) {
+    const int sb_ = (int)(lg % nsb), e_ = (int)(lg / nsb);
+    const int s0_ = sb_ * SPAN;
+    const int fmax_ = a.Ns + a.T - 2 - s0_;
+    const __amdgpu_buffer_rsrc_t rin_ = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(a.in + (long)s0_ * spec + (long)e_ * N), (short)0, 0x7fffffff, kBufferWord3);
+    const int tid3 = opaque(tid);
+    const int wv = __builtin_amdgcn_readfirstlane(tid3 >> 6), lane = tid3 & 63;
+    constexpr int KPF = N * 2 / 1024;  // wave-instructions per frame
+    for (int c = wv; c < NF * KPF; c += C::THREADS / 64) {
+      const int f = c / KPF, k = c % KPF;
+      const int fr = f < fmax_ ? f : fmax_;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rin_, (__attribute__((address_space(3))) void *)((char *)frb + (long)c * 1024), 16,
+          lane * 16 + k * 1024, (int)(2 * (long)fr * spec), 0, 0);
+    }
+  };
+  if constexpr (DMA) {
+    if (logical_of(0) < nitems) dma_frames(logical_of(0));
+  }
   // |x|^2 of this thread's channel t + m TPB added to its power sum
   auto add_power = [&](int m, float xr, float xi) {
     float &p = pwl[q * N + t + m * TPB];
     p = __builtin_fmaf(xr, xr, __builtin_fmaf(xi, xi, p));
   };
   for (long base = 0; base < nitems; base += G) {
-    // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
-    // spectrum blocks of one input: shared FIR frames, adjacent output pieces)
-    // go to the same XCD.
-    const long logical = (G % 8 == 0) ? base + (b % 8) * (G / 8) + b / 8 : base + b;
+    const long logical = logical_of(base);
     if (logical >= nitems) break;
     const int sb = (int)(logical % nsb);
     const int e = (int)(logical / nsb);
@@ -412,6 +481,14 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(a.in + (long)s0 * spec + (long)e * N), (short)0, 0x7fffffff, kBufferWord3);
     const int fmax = a.Ns + a.T - 2 - s0;  // last frame of the input, relative to s0
+    if constexpr (DMA) {
+      // this item's frames (loaded during the previous item, or before the
+      // loop) are in LDS: every wave's DMA done, then visible to all
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    }
     int twoff = 0;
     asm volatile("" : "+s"(twoff));
     const PassTw<P> tws{twl + twoff};
@@ -436,8 +513,11 @@ fengine_persistent_kernel(FArgs a, long nitems) {
 #pragma unroll
       for (int f = 0; f < SB + T - 1; f++) {
         const int fr = SB * q + f < fmax ? SB * q + f : fmax;
-        const unsigned short u =
-            __builtin_amdgcn_raw_buffer_load_b16(rin, 2 * (tl + m * TPB), (int)(2 * (long)fr * spec), 0);
+        unsigned short u;
+        if constexpr (DMA)  // clamped when copied
+          u = ((const unsigned short *)frb)[(SB * q + f) * N + tl + m * TPB];
+        else
+          u = __builtin_amdgcn_raw_buffer_load_b16(rin, 2 * (tl + m * TPB), (int)(2 * (long)fr * spec), 0);
         // (I, Q) = sign-extended low and high bytes
         if constexpr (FeTuning<SPLIT>::byte1)
           x[f] = make_float2((float)(signed char)u, (float)((int)((unsigned)u << 16) >> 24));
@@ -458,8 +538,30 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     // transforms, one at a time, all SB of them: inside `if (sl < scount)`
     // the FIR sums of the later spectra would be sunk into those branches,
     // keeping every frame's samples live until the last transform
+    if constexpr (DMA) {
+      // every wave's frame reads done: the frame area becomes exchange
+      // buffers; sub-block 0 uses frb slots 0 and 1, sub-block 1 slot 2 and xb
+      __syncthreads();
+      float2 *l0 = frb + 2 * q * C::FFT_LDS;
+      float2 *l1 = q == 0 ? frb + C::FFT_LDS : xb;
 #pragma unroll
-    for (int sl = 0; sl < SB; sl++) {
+      for (int pr = 0; pr < SB; pr += 2) {
+        float2(&xx)[2][P] = *reinterpret_cast<float2(*)[2][P]>(&acc[pr]);
+        fft_block_pair<N, P, false>(xx, opaque(t), l0, l1, tw2);
+        if constexpr (FeTuning<SPLIT>::pin) {
+#pragma unroll
+          for (int v = 0; v < 2; v++)
+#pragma unroll
+            for (int m = 0; m < P; m++) asm volatile("" : "+v"(xx[v][m].x), "+v"(xx[v][m].y));
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __syncthreads();  // every exchange read done: the frame area is free
+      const long nx = logical_of(base + G);
+      if (nx < nitems) dma_frames(nx);
+    }
+#pragma unroll
+    for (int sl = 0; sl < (DMA ? 0 : SB); sl++) {
       fft_block_tw<N, P, false>(acc[sl], opaque(t), fbuf, tws);
       if constexpr (FeTuning<SPLIT>::pin) {
 #pragma unroll
@@ -552,7 +654,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       const int tid2 = opaque(tid);
       const int s1 = (tid2 % CPR) * SPC;  // the same for every v
       const int row0 = tid2 / CPR;
-      const float2 *src0 = buf + stage_idx<C>(row0, s1);  // the lane's chunk; rows row0 + v*RV share its swizzle
+      const float2 *src0 = sbuf + stage_idx<C>(row0, s1);  // the lane's chunk; rows row0 + v*RV share its swizzle
       long xoff, ibase;
       if constexpr (XL == XL_ROWS) {
         xoff = (long)row0 * a.xg.cp + s1;
@@ -570,15 +672,15 @@ fengine_persistent_kernel(FArgs a, long nitems) {
       float f8scale = 1.f;  // split 3: 2^eb of the lane's block of 16 spectra
 #pragma unroll
       for (int m0 = 0; m0 < P; m0 += MR) {
-        __syncthreads();
+        stage_barrier<DMA>();
         tl = opaque(t);
 #pragma unroll
         for (int mr = 0; mr < MR; mr++)
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
-            buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
+            sbuf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
                 (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], split_sc) : acc[sl][m0 + mr];
-        __syncthreads();
+        stage_barrier<DMA>();
         if constexpr (SPLIT == 3) {
           if (m0 == 0) {  // the waves' maxima are visible: each block's exponent
             constexpr int WPS = TPB / 64;  // waves per sub-block
@@ -634,15 +736,15 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     } else {  // the offsets recomputed in every round
 #pragma unroll
       for (int m0 = 0; m0 < P; m0 += MR) {
-        __syncthreads();
+        stage_barrier<DMA>();
         tl = opaque(t);
 #pragma unroll
         for (int mr = 0; mr < MR; mr++)
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
-            buf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
+            sbuf[stage_idx<C>(mr * TPB + tl, SB * q + sl)] =
                 (SPLIT == 1) ? split_sample(acc[sl][m0 + mr], split_sc) : acc[sl][m0 + mr];
-        __syncthreads();
+        stage_barrier<DMA>();
         const int tid2 = opaque(tid);
 #pragma unroll
         for (int v = 0; v < (MR * TPB * CPR) / C::THREADS; v++) {
@@ -662,7 +764,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
             else
               o = ((long)blk * N + k) * a.xg.cp + (e * B + in_blk);
           }
-          const float2 *src_ = buf + stage_idx<C>(row, s1);
+          const float2 *src_ = sbuf + stage_idx<C>(row, s1);
           if constexpr (SPLIT == 2) {  // 4-byte FP16 samples, 4 per chunk
             unsigned *d4 = (unsigned *)a.out + o;
             const unsigned h0 = half_sample(src_[0], split_sc), h1 = half_sample(src_[1], split_sc);
@@ -688,7 +790,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         }
       }
     }
-    __syncthreads();  // the staging buffer is the next item's exchange buffers
+    stage_barrier<DMA>();  // the staging buffer is the next item's exchange buffers
   }
   if constexpr (SPLIT == 2) {
     // the workgroup's channel powers: its sub-blocks' sums added in the

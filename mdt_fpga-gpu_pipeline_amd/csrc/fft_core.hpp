@@ -287,6 +287,58 @@ __device__ __forceinline__ void fft_block_multi(float2 (&x)[NX][P], int t, float
   fft_stage_multi<N, P, INV, 0, NX>(x, t, lds, tw);
 }
 
+// Two independent transforms advanced together, their exchange buffers at two
+// separate places (l0, l1: lds_entries<N>() float2 each): one pair of
+// barriers per exchange round for both, and the LDS reads of the second
+// overlap the butterflies of the first.  Same arithmetic as fft_block_tw.
+template <int N, int P, bool INV, int PASS, typename TW>
+__device__ __forceinline__ void fft_stage_pair(float2 (&x)[2][P], int t, float2 *l0, float2 *l1,
+                                               const TW &tw) {
+  constexpr int L = full_passes(N, P);
+  constexpr int RL = N / ipow_c(P, L);
+  constexpr int NPASS = L + (RL > 1 ? 1 : 0);
+  constexpr int R = (PASS < L) ? P : RL;
+  constexpr int NS = ipow_c(P, PASS);
+  constexpr int TPB = N / P;
+  static_assert(TPB % 16 == 0, "strided exchange reads");
+  if constexpr (PASS > 0) {
+    fft_exchange_sync<TPB>();
+    const float2 *r0 = l0 + lpad(t), *r1 = l1 + lpad(t);
+#pragma unroll
+    for (int m = 0; m < P; m++) x[0][m] = r0[m * (TPB + TPB / 16)];
+#pragma unroll
+    for (int m = 0; m < P; m++) x[1][m] = r1[m * (TPB + TPB / 16)];
+    fft_exchange_sync<TPB>();
+  }
+  fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x[0], t, l0, tw);
+  fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x[1], t, l1, tw);
+  if constexpr (PASS + 1 < NPASS) fft_stage_pair<N, P, INV, PASS + 1>(x, t, l0, l1, tw);
+}
+template <int N, int P, bool INV, typename TW>
+__device__ __forceinline__ void fft_block_pair(float2 (&x)[2][P], int t, float2 *l0, float2 *l1,
+                                               const TW &tw) {
+  fft_stage_pair<N, P, INV, 0>(x, t, l0, l1, tw);
+}
+
+// Twiddles of the passes before the last from the per-pass LDS table (as
+// PassTw), those of the last radix-P pass from registers: there NS = N/P =
+// the threads per transform, so thread t's butterfly uses W_N^{t r}, the
+// same every transform.
+template <int N, int P>
+struct LastRegTw {
+  const float2 *tab;  // LDS, passes 1 .. L-2
+  float2 w[P];        // W_N^{t r}, r = 0 .. P-1
+  template <int N_, int NS, int R>
+  __device__ __forceinline__ float2 get(int jm, int r) const {
+    if constexpr (NS * P == N && R == P) {
+      return w[r];
+    } else {
+      constexpr int pass = ilog2_c(NS) / ilog2_c(P);
+      return tab[pass_tw_offset<P>(pass) + r * NS + jm];
+    }
+  }
+};
+
 // Transform one N-point vector held as x[m] = element t + m*(N/P).  All
 // threads of the block must call this the same number of times (it contains
 // __syncthreads when the transform spans more than one wave).  lds:
