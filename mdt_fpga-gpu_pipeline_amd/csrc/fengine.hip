@@ -273,20 +273,52 @@ constexpr int fe_waves_per_simd() {
   return (GAIN || fft_points(N) == 16) ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
 }
 
-// Frame prefetch (4096 channels x 8 taps, FP32 and FP16 hi/lo outputs): the
-// next item's 15 input frames (120 KB of int8 (I, Q)) are copied into LDS by
-// buffer-to-LDS DMA while this item is corner-turned and stored, and the FIR
-// reads its frames from LDS.  The FIR's 88 two-byte global loads per thread
-// and item had made it wait on memory latency several times per item (C3
-// F-engine 8.33 ms; with the frames prefetched 7.0 ms, same bits).  The
-// frame area doubles as three of the four exchange buffers of the two
-// transforms each sub-block advances together; the fourth holds the
-// corner-turn staging, so the DMA never targets LDS in use.  The last pass's
-// twiddles are held in registers (the LDS table keeps passes 1 and 2).
+// Frame prefetch by LDS DMA: the next item's NF = SPAN + T - 1 input frames
+// (int8 (I, Q), [frame][channel]) are copied into LDS by buffer-to-LDS DMA
+// while this item is corner-turned and stored, and the FIR reads its frames
+// from LDS.  The FIR's two-byte global loads (88 per thread and item at 4096
+// channels) had made it wait on memory latency several times per item (C3
+// F-engine 8.33 ms; with the frames prefetched 7.06 ms, same bits).  During
+// the transforms the frame area holds the exchange buffers (the frames are
+// read by then); the corner-turn staging has its own area, so the DMA never
+// targets LDS in use.  At 4096 channels (8 taps, FP32 and FP16 hi/lo
+// outputs) each sub-block advances two transforms together: three of the
+// four exchange buffers sit in the frame area, the fourth beside the staging,
+// and the last pass's twiddles are held in registers (the LDS table keeps
+// passes 1 and 2).  At 1024 and 2048 channels the frame area holds every
+// sub-block's exchange buffer; the staging rounds per barrier pair (MR) are
+// as many as the remaining LDS allows.  Served where it measured faster
+// (F-engine alone, same bits; 64 inputs x 8192 spectra at 1024 channels, 40
+// and 64 inputs x 4096 spectra at 2048): 1024 channels every output
+// (0.85-0.97x the time); 2048 channels the FP16 and FP8 outputs (0.94x at 4
+// taps, 0.34x at 8 taps, where the taps do not fit in LDS and the FIR's tap
+// loads queued behind its frame loads); not the FP32 output at 2048 channels
+// (1.12x: two waves per SIMD spill beside the larger staging rounds) nor 512
+// channels (1.07-1.23x: those items are short, the DMA is not hidden).
 template <int N, int T, int SPLIT, bool GAIN>
-constexpr bool fe_dma_mode() {
-  return N == 4096 && T == 8 && !GAIN && (SPLIT == 0 || SPLIT == 1);
-}
+struct FeDma {
+  using C = PCfg<N, SPLIT, GAIN>;
+  static constexpr int NF = C::SPAN + T - 1;
+  static constexpr int FRB = NF * N / 4;  // frame area, float2
+  static constexpr bool PAIR = N == 4096;
+  static constexpr int L = full_passes(N, C::P);
+  static constexpr int TWN = PAIR ? pass_tw_offset<C::P>(L - 1) : C::TWN;
+  static constexpr bool TAPS = T * N * 4 <= 32768;
+  static constexpr int ROWS = C::TPB * C::ROWW;  // staging float2 per channel round
+  static constexpr int XB_MIN = PAIR ? C::FFT_LDS : 0;
+  static constexpr int FIXED = TWN * 8 + FRB * 8 + (TAPS ? T * N * 4 : 0) + (SPLIT == 2 ? C::Q * N * 4 : 0) +
+                               (SPLIT == 3 ? C::THREADS / 16 : 0) + 64;
+  static constexpr int xb(int mr) { return XB_MIN > mr * ROWS ? XB_MIN : mr * ROWS; }
+  static constexpr bool fits(int mr) {
+    return mr <= C::MR && (mr * C::TPB * C::CPR) % C::THREADS == 0 && xb(mr) * 8 + FIXED <= 163840;
+  }
+  static constexpr int MR = fits(4) ? 4 : (fits(2) ? 2 : 1);
+  static constexpr int XB = xb(MR);
+  static constexpr bool ENABLED =
+      fits(MR) && (PAIR ? (T == 8 && !GAIN && SPLIT <= 1 && C::Q == 2 && FRB >= 3 * C::FFT_LDS &&
+                           C::FFT_LDS >= ROWS && N / ipow_c(C::P, L) == 1)
+                        : ((N == 1024 || (N == 2048 && SPLIT != 0)) && FRB >= C::Q * C::FFT_LDS));
+};
 
 // s_barrier after this wave's LDS accesses completed, without the release
 // fence of __syncthreads(): that fence also waits for every load in flight,
@@ -370,18 +402,19 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   // loads are its frames and each wait counts down through them in order;
   // read through L1/L2 otherwise, where a tap load queued behind frame loads
   // waits for all of them
-  constexpr bool TAPS_LDS = T * N * 4 <= 32768 && (C::TWN + C::BUF) * 8 + T * N * 4 <= 150 * 1024;
-  constexpr bool DMA = fe_dma_mode<N, T, SPLIT, GAIN>();
+  using D = FeDma<N, T, SPLIT, GAIN>;
+  constexpr bool DMA = D::ENABLED;                       // frames prefetched into LDS
+  constexpr bool PAIR = DMA && D::PAIR;                  // two transforms per sub-block
+  constexpr bool TAPS_LDS =
+      DMA ? D::TAPS : (T * N * 4 <= 32768 && (C::TWN + C::BUF) * 8 + T * N * 4 <= 150 * 1024);
   constexpr int NF = SPAN + T - 1;                       // input frames per item
-  constexpr int TWN = DMA ? pass_tw_offset<P>(L - 1) : C::TWN;
-  constexpr int MR = DMA ? 1 : C::MR;                    // staging rounds per barrier pair
-  static_assert(!DMA || (RL == 1 && NF * N / 4 >= 3 * C::FFT_LDS && C::FFT_LDS >= TPB * ROWW && C::Q == 2),
-                "frame area holds three exchange buffers, the fourth the staging");
+  constexpr int TWN = DMA ? D::TWN : C::TWN;
+  constexpr int MR = DMA ? D::MR : C::MR;                // staging rounds per barrier pair
   __shared__ float2 twl[TWN];
   __shared__ float2 buf[DMA ? 1 : C::BUF];
-  __shared__ float2 frb[DMA ? NF * N / 4 : 1];  // DMA: the item's frames, [f][channel] int8 (I, Q)
-  __shared__ float2 xb[DMA ? C::FFT_LDS : 1];   // DMA: fourth exchange buffer and the staging
-  float2 *const sbuf = DMA ? xb : buf;         // corner-turn staging
+  __shared__ float2 frb[DMA ? D::FRB : 1];  // DMA: the item's frames, [f][channel] int8 (I, Q)
+  __shared__ float2 xb[DMA ? D::XB : 1];    // DMA: the staging (PAIR: and the fourth exchange buffer)
+  float2 *const sbuf = DMA ? xb : buf;     // corner-turn staging
   __shared__ float wmax[SPLIT == 3 ? C::THREADS / 64 : 1];  // split 3: each wave's largest |re|, |im|
   __shared__ v4u wtl[TAPS_LDS ? T * N / 4 : 1];
   // split 2: the channel powers summed in LDS, one row per sub-block, each
@@ -417,8 +450,8 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     const int rel = idx - off, r = rel / ns, jm = rel % ns;
     twl[idx] = a.tw[jm * r * (N / (ns * R))];
   }
-  LastRegTw<N, P> tw2{twl, {}};  // DMA mode: the last pass's twiddles in registers
-  if constexpr (DMA) {
+  LastRegTw<N, P> tw2{twl, {}};  // PAIR: the last pass's twiddles in registers
+  if constexpr (PAIR) {
 #pragma unroll
     for (int r = 0; r < P; r++) tw2.w[r] = a.tw[t * r];
   }
@@ -430,7 +463,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   const long spec = (long)a.NN * N;
   const __amdgpu_buffer_rsrc_t rtap =
       __builtin_amdgcn_make_buffer_rsrc((void *)a.Wt, (short)0, T * N * 4, kBufferWord3);
-  float2 *fbuf = buf + q * C::FFT_LDS;  // this sub-block's exchange buffer
+  float2 *fbuf = (DMA ? frb : buf) + q * C::FFT_LDS;  // this sub-block's exchange buffer
   // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
   // spectrum blocks of one input: shared FIR frames, adjacent output pieces)
   // go to the same XCD.
@@ -538,10 +571,11 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     // transforms, one at a time, all SB of them: inside `if (sl < scount)`
     // the FIR sums of the later spectra would be sunk into those branches,
     // keeping every frame's samples live until the last transform
-    if constexpr (DMA) {
-      // every wave's frame reads done: the frame area becomes exchange
-      // buffers; sub-block 0 uses frb slots 0 and 1, sub-block 1 slot 2 and xb
-      __syncthreads();
+    // DMA: every wave's frame reads done before the frame area becomes the
+    // exchange buffers
+    if constexpr (DMA) __syncthreads();
+    if constexpr (PAIR) {
+      // sub-block 0 uses frb slots 0 and 1, sub-block 1 slot 2 and xb
       float2 *l0 = frb + 2 * q * C::FFT_LDS;
       float2 *l1 = q == 0 ? frb + C::FFT_LDS : xb;
 #pragma unroll
@@ -556,12 +590,9 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      __syncthreads();  // every exchange read done: the frame area is free
-      const long nx = logical_of(base + G);
-      if (nx < nitems) dma_frames(nx);
     }
 #pragma unroll
-    for (int sl = 0; sl < (DMA ? 0 : SB); sl++) {
+    for (int sl = 0; sl < (PAIR ? 0 : SB); sl++) {
       fft_block_tw<N, P, false>(acc[sl], opaque(t), fbuf, tws);
       if constexpr (FeTuning<SPLIT>::pin) {
 #pragma unroll
@@ -578,6 +609,11 @@ fengine_persistent_kernel(FArgs a, long nitems) {
           add_power(m, xr, xi);
         }
       }
+    }
+    if constexpr (DMA) {
+      __syncthreads();  // every exchange read done: the frame area is free
+      const long nx = logical_of(base + G);
+      if (nx < nitems) dma_frames(nx);
     }
 
     if constexpr (GAIN) {
