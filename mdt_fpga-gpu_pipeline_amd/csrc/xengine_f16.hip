@@ -282,9 +282,80 @@ __device__ __forceinline__ void wc_block(half8 Ih, half8 Il, half8 Ph, half8 Pl,
   aI = __builtin_amdgcn_mfma_f32_32x32x16_f16(Pl, Jh, aI, 0, 0, 0);
 }
 
-// one chunk (2 K-steps) of an A wave: I row `irow`, J rows 32j + l32, j = 0..4
-__device__ __forceinline__ void wc_chunk_a(const char *bufp, int irow, int l32, int kh, f32x16 (&accR)[WC_A],
-                                           f32x16 (&accI)[WC_A]) {
+// Diagonal 32 x 32 blocks as three 16 x 16 blocks: rows 0-15 x columns 0-15,
+// rows 16-31 x columns 0-15 and rows 16-31 x columns 16-31 (the fourth lies
+// above the diagonal), on v_mfma_f32_16x16x32_f16, which takes a chunk's 16
+// samples in one step: 3/4 of the block's MFMA work.  Lane l supplies row
+// l % 16 and samples 4 (l / 16) .. +3 of the chunk; result register r holds
+// D[4 (l / 16) + r][l % 16].
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void wc_block16(half8 Ih, half8 Il, half8 Ph, half8 Pl, half8 Jh, half8 Jl, f32x4 &aR,
+                                           f32x4 &aI) {
+  aR = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ih, Jh, aR, 0, 0, 0);
+  aI = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ph, Jh, aI, 0, 0, 0);
+  aR = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ih, Jl, aR, 0, 0, 0);
+  aI = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ph, Jl, aI, 0, 0, 0);
+  aR = __builtin_amdgcn_mfma_f32_16x16x32_f16(Il, Jh, aR, 0, 0, 0);
+  aI = __builtin_amdgcn_mfma_f32_16x16x32_f16(Pl, Jh, aI, 0, 0, 0);
+}
+// one chunk of diagonal block R (rows 32 R .. 32 R + 31)
+__device__ __forceinline__ void wc_diag16(const char *bufp, int R, int lane, f32x4 (&dR)[3], f32x4 (&dI)[3]) {
+  const int row0 = R * 32 + (lane & 15), row1 = row0 + 16;
+  const int koff = (lane >> 4) * 16;
+  const half8 A0h = *(const half8 *)(bufp + row0 * WC_ROWB + koff);
+  const half8 A0l = *(const half8 *)(bufp + WC_PLANE + row0 * WC_ROWB + koff);
+  const half8 A1h = *(const half8 *)(bufp + row1 * WC_ROWB + koff);
+  const half8 A1l = *(const half8 *)(bufp + WC_PLANE + row1 * WC_ROWB + koff);
+  const half8 P0h = rot_conj(A0h), P0l = rot_conj(A0l), P1h = rot_conj(A1h), P1l = rot_conj(A1l);
+  wc_block16(A0h, A0l, P0h, P0l, A0h, A0l, dR[0], dI[0]);
+  wc_block16(A1h, A1l, P1h, P1l, A0h, A0l, dR[1], dI[1]);
+  wc_block16(A1h, A1l, P1h, P1l, A1h, A1l, dR[2], dI[2]);
+}
+// V of the three 16 x 16 blocks of diagonal block R (as tri_block_out)
+__device__ __forceinline__ void wc_diag16_out(const XArgs &g, int batch, float oscale, int R, int lane,
+                                              const f32x4 (&dR)[3], const f32x4 (&dI)[3]) {
+  float2 *__restrict__ C = g.Cbase;
+  double2 *__restrict__ acc = (double2 *)g.acc64;
+  const long cb = (long)batch * g.Cbatch;
+  const int NN = g.m;
+#pragma unroll
+  for (int sb = 0; sb < 3; sb++) {
+    const int a0 = R * 32 + (sb == 0 ? 0 : 16) + 4 * (lane >> 4);
+    const int b = R * 32 + (sb == 2 ? 16 : 0) + (lane & 15);
+    long off[4];
+    bool ok[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int a = a0 + r;
+      ok[r] = a < NN && b < NN && a >= b;
+      off[r] = cb + (ok[r] ? (long)a * (a + 1) / 2 + b : 0);
+    }
+    if (acc) {
+      double2 old[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) old[r] = acc[off[r]];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (ok[r]) {
+          const float vr = oscale * dR[sb][r], vi = -oscale * dI[sb][r];
+          C[off[r]] = make_float2(vr, vi);
+          acc[off[r]] = make_double2(old[r].x + (double)vr, old[r].y + (double)vi);
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (ok[r]) C[off[r]] = make_float2(oscale * dR[sb][r], -oscale * dI[sb][r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// one chunk (2 K-steps) of an A wave: I row `irow`, J rows 32j + l32, j = 0..4;
+// DIAG (wave 3, row 4): block (4, 4) is diagonal, three 16 x 16 blocks
+template <bool DIAG>
+__device__ __forceinline__ void wc_chunk_a(const char *bufp, int irow, int l32, int kh, int lane,
+                                           f32x16 (&accR)[WC_A], f32x16 (&accI)[WC_A], f32x4 (&dR)[3],
+                                           f32x4 (&dI)[3]) {
 #pragma unroll
   for (int ks = 0; ks < WC_KC / 8; ks++) {
     const int koff = ks * 32 + kh * 16;
@@ -292,13 +363,55 @@ __device__ __forceinline__ void wc_chunk_a(const char *bufp, int irow, int l32, 
     const half8 Il = *(const half8 *)(bufp + WC_PLANE + irow * WC_ROWB + koff);
     const half8 Ph = rot_conj(Ih), Pl = rot_conj(Il);
 #pragma unroll
-    for (int j = 0; j < WC_A; j++) {
+    for (int j = 0; j < (DIAG ? WC_A - 1 : WC_A); j++) {
       const int jrow = j * 32 + l32;
       const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
       const half8 Jl = *(const half8 *)(bufp + WC_PLANE + jrow * WC_ROWB + koff);
       wc_block(Ih, Il, Ph, Pl, Jh, Jl, accR[j], accI[j]);
     }
   }
+  if constexpr (DIAG) wc_diag16(bufp, WC_A - 1, lane, dR, dI);
+}
+
+// A wave: the five blocks (r, 0..4) of row r over all chunks, then their
+// epilogue; r = 4 (DIAG) has the diagonal block (4, 4)
+template <bool DIAG>
+__device__ __forceinline__ void wc_run_a(char *lds, const float2 *base, const XArgs &g, int batch, float oscale,
+                                         int NN, int K, int r, int l32, int kh, int lane) {
+  constexpr int NO = DIAG ? WC_A - 1 : WC_A;
+  int ra[NO], rb[NO];
+  f32x16 accR[WC_A], accI[WC_A];
+  f32x4 dR[3], dI[3];
+#pragma unroll
+  for (int b = 0; b < WC_A; b++) {
+    if (b < NO) {
+      ra[b] = r;
+      rb[b] = b;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      accR[b][q] = 0.f;
+      accI[b][q] = 0.f;
+    }
+  }
+#pragma unroll
+  for (int sb = 0; sb < 3; sb++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      dR[sb][q] = 0.f;
+      dI[sb][q] = 0.f;
+    }
+  const int irow = r * 32 + l32;
+  wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K,
+          [&](const char *bufp) { wc_chunk_a<DIAG>(bufp, irow, l32, kh, lane, accR, accI, dR, dI); });
+  f32x16 oR[NO], oI[NO];
+#pragma unroll
+  for (int b = 0; b < NO; b++) {
+    oR[b] = accR[b];
+    oI[b] = accI[b];
+  }
+  wc_epilogue(g, batch, oscale, ra, rb, l32, kh, oR, oI);
+  if constexpr (DIAG) wc_diag16_out(g, batch, oscale, WC_A - 1, lane, dR, dI);
 }
 
 // The B waves' blocks (row, column) as constants, wave 4 + W: consecutive
@@ -326,7 +439,7 @@ __device__ __forceinline__ void wc_b_step(const char *bufp, int l32, int koff, h
       Pl = rot_conj(Il);
     }
     if constexpr (Cc == R) {
-      wc_block(Ih, Il, Ph, Pl, Ih, Il, accR[B], accI[B]);
+      // diagonal: three 16 x 16 blocks per chunk (wc_chunk_b)
     } else {
       const int jrow = Cc * 32 + l32;
       const half8 Jh = *(const half8 *)(bufp + jrow * WC_ROWB + koff);
@@ -339,13 +452,27 @@ __device__ __forceinline__ void wc_b_step(const char *bufp, int l32, int koff, h
 
 // one chunk of B wave 4 + W
 template <int W>
-__device__ __forceinline__ void wc_chunk_b(const char *bufp, int l32, int kh, f32x16 (&accR)[WC_B],
-                                           f32x16 (&accI)[WC_B]) {
+constexpr int wc_b_ndiag() {
+  int n = 0;
+  for (int b = 0; b < WC_B; b++) n += WcBw<W>::r[b] == WcBw<W>::c[b];
+  return n;
+}
+template <int W>
+constexpr int wc_b_diag_row(int d) {  // row of wave 4 + W's d-th diagonal block
+  for (int b = 0; b < WC_B; b++)
+    if (WcBw<W>::r[b] == WcBw<W>::c[b] && d-- == 0) return WcBw<W>::r[b];
+  return -1;
+}
+template <int W>
+__device__ __forceinline__ void wc_chunk_b(const char *bufp, int l32, int kh, int lane, f32x16 (&accR)[WC_B],
+                                           f32x16 (&accI)[WC_B], f32x4 (&dR)[2][3], f32x4 (&dI)[2][3]) {
 #pragma unroll
   for (int ks = 0; ks < WC_KC / 8; ks++) {
     half8 Ih, Il, Ph, Pl;
     wc_b_step<W, 0>(bufp, l32, ks * 32 + kh * 16, Ih, Il, Ph, Pl, accR, accI);
   }
+#pragma unroll
+  for (int d = 0; d < wc_b_ndiag<W>(); d++) wc_diag16(bufp, wc_b_diag_row<W>(d), lane, dR[d], dI[d]);
 }
 
 // V for blocks (ra[b], rb[b]): D[i][j] = V[j][i], so accumulator register r of
@@ -390,21 +517,52 @@ __device__ __forceinline__ void wc_loop(char *lds, const float2 *base, long ld, 
 // B wave 4 + W: its four blocks over all chunks, then their epilogue
 template <int W>
 __device__ __forceinline__ void wc_run_b(char *lds, const float2 *base, const XArgs &g, int batch, float oscale,
-                                         int NN, int K, int l32, int kh) {
-  int ra[WC_B], rb[WC_B];
-  f32x16 accR[WC_B], accI[WC_B];
+                                         int NN, int K, int l32, int kh, int lane) {
+  constexpr int ND = wc_b_ndiag<W>(), NO = WC_B - ND;  // diagonal and other blocks
+  int ra[NO], rb[NO];
+  f32x16 accR[WC_B], accI[WC_B];  // diagonal entries unused
+  f32x4 dR[2][3], dI[2][3];
+  {
+    int o = 0;
 #pragma unroll
-  for (int b = 0; b < WC_B; b++) {
-    ra[b] = WcBw<W>::r[b];
-    rb[b] = WcBw<W>::c[b];
+    for (int b = 0; b < WC_B; b++) {
+      if (WcBw<W>::r[b] != WcBw<W>::c[b]) {
+        ra[o] = WcBw<W>::r[b];
+        rb[o] = WcBw<W>::c[b];
+        o++;
+      }
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      accR[b][q] = 0.f;
-      accI[b][q] = 0.f;
+      for (int q = 0; q < 16; q++) {
+        accR[b][q] = 0.f;
+        accI[b][q] = 0.f;
+      }
     }
   }
-  wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_b<W>(bufp, l32, kh, accR, accI); });
-  wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
+#pragma unroll
+  for (int d = 0; d < 2; d++)
+#pragma unroll
+    for (int sb = 0; sb < 3; sb++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        dR[d][sb][q] = 0.f;
+        dI[d][sb][q] = 0.f;
+      }
+  wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K,
+          [&](const char *bufp) { wc_chunk_b<W>(bufp, l32, kh, lane, accR, accI, dR, dI); });
+  f32x16 oR[NO], oI[NO];
+  {
+    int o = 0;
+#pragma unroll
+    for (int b = 0; b < WC_B; b++)
+      if (WcBw<W>::r[b] != WcBw<W>::c[b]) {
+        oR[o] = accR[b];
+        oI[o] = accI[b];
+        o++;
+      }
+  }
+  wc_epilogue(g, batch, oscale, ra, rb, l32, kh, oR, oI);
+#pragma unroll
+  for (int d = 0; d < ND; d++) wc_diag16_out(g, batch, oscale, wc_b_diag_row<W>(d), lane, dR[d], dI[d]);
 }
 
 __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs g, float oscale) {
@@ -425,28 +583,16 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
 
   // blocks past NN (NN < 256) are multiplied out of zero rows and not stored
   if (wave < 4) {
-    const int r = 7 - wave;
-    int ra[WC_A], rb[WC_A];
-    f32x16 accR[WC_A], accI[WC_A];
-#pragma unroll
-    for (int b = 0; b < WC_A; b++) {
-      ra[b] = r;
-      rb[b] = b;
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        accR[b][q] = 0.f;
-        accI[b][q] = 0.f;
-      }
-    }
-    const int irow = r * 32 + l32;
-    wc_loop(lds, base, g.ldI, g.xlgB, g.xbp, NN, K, [&](const char *bufp) { wc_chunk_a(bufp, irow, l32, kh, accR, accI); });
-    wc_epilogue(g, batch, oscale, ra, rb, l32, kh, accR, accI);
+    if (wave == 3)
+      wc_run_a<true>(lds, base, g, batch, oscale, NN, K, 4, l32, kh, lane);
+    else
+      wc_run_a<false>(lds, base, g, batch, oscale, NN, K, 7 - wave, l32, kh, lane);
   } else {
     switch (wave) {
-      case 4: wc_run_b<0>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
-      case 5: wc_run_b<1>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
-      case 6: wc_run_b<2>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
-      default: wc_run_b<3>(lds, base, g, batch, oscale, NN, K, l32, kh); break;
+      case 4: wc_run_b<0>(lds, base, g, batch, oscale, NN, K, l32, kh, lane); break;
+      case 5: wc_run_b<1>(lds, base, g, batch, oscale, NN, K, l32, kh, lane); break;
+      case 6: wc_run_b<2>(lds, base, g, batch, oscale, NN, K, l32, kh, lane); break;
+      default: wc_run_b<3>(lds, base, g, batch, oscale, NN, K, l32, kh, lane); break;
     }
   }
 }
