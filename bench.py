@@ -14,8 +14,10 @@ the host-inclusive streaming rates, int8 stacks and the reference's SNAP
 packets (--no-stream), the CPU port and the FP64 error (--no-cpu), the
 512-input FP8 wide-band shape (--no-fp8), the reference's drop-in call
 sequence at the MDT station's shape with the FFT at lengths that are not
-powers of two (--no-station; --station-only runs that leg alone) and the
-gain/delay calibration shape (--no-calib).
+powers of two (--no-station; --station-only runs that leg alone), the
+gain/delay calibration shape (--no-calib) and the HBM read / write / copy
+rates this device delivers to a streaming kernel, with the F-engine's time
+against the floor they set (--no-hbm-probe).
 """
 import argparse
 import json
@@ -521,6 +523,35 @@ def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, er
     return cpu, err
 
 
+
+def hbm_probe_leg(torch, mdtfx, nbytes=2684354560, grids=(1024, 2048, 4096, 8192, 16384), reps=10):
+    """What this device's HBM delivers to a streaming kernel with 16-byte lanes
+    (the library's probe kernels): reads alone, streaming writes alone, and a
+    copy (read + write), best over a few grid sizes, each timed with HIP
+    events over `reps` passes of `nbytes`."""
+    log("bench: HBM probe (read, write, copy ceilings)")
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    best = {}
+    for kind, name, moved in ((mdtfx.PROBE_READ, "read", 1), (mdtfx.PROBE_WRITE, "write", 1),
+                              (mdtfx.PROBE_COPY, "copy", 2)):
+        rates = {}
+        for g in grids:
+            mdtfx.hbm_probe(kind, src, dst, g)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                mdtfx.hbm_probe(kind, src, dst, g)
+            e1.record()
+            torch.cuda.synchronize()
+            rates[g] = moved * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        g = max(rates, key=rates.get)
+        best[name] = {"GB/s": rates[g], "grid": g, "frac_of_8TBs": rates[g] / HBM_PEAK_GBS}
+    del src, dst
+    return {"bytes_per_pass": nbytes, "unit": "GB/s", **best,
+            "definition": "best over grids of 256-thread workgroups; read and copy count every byte moved"}
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -535,6 +566,7 @@ def main():
     ap.add_argument("--no-calib", action="store_true", help="skip the calibration leg")
     ap.add_argument("--station-only", action="store_true",
                     help="only run the drop-in station leg and print its JSON (for per-kernel profiles)")
+    ap.add_argument("--no-hbm-probe", action="store_true", help="skip the HBM read/write/copy ceiling leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -654,6 +686,17 @@ def main():
         out["calibration"] = calib_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), not args.no_cpu)
     if not args.no_stream:
         out["packet_stream_host_inclusive"] = packet_stream_leg(np, torch, mdtfx, 4)
+    if not args.no_hbm_probe:
+        probe = hbm_probe_leg(torch, mdtfx)
+        # the dominant kernel against what this device measured: its reads at
+        # the read rate plus its writes at the write rate
+        fe = out["kernels"]["fengine"]
+        rd = NN * (NS + T - 1) * N * 2 + T * N * 4
+        wr = fe["algorithmic_bytes"] - rd
+        floor_ms = (rd / probe["read"]["GB/s"] + wr / probe["write"]["GB/s"]) / 1e6
+        probe["fengine_floor_ms"] = floor_ms
+        probe["fengine_frac_of_measured_floor"] = floor_ms / fe["ms"]
+        out["hbm_measured_ceiling"] = probe
     print(json.dumps(out), flush=True)
 
 

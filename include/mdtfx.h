@@ -457,6 +457,18 @@ typedef struct {
 int mdt_synth_i8(const mdt_synth_params *p, int NpolsxNelements, int Nchannels, int nframes,
                  int8_t *out, hipStream_t stream);
 
+/* ---------------------------------------------------------------- HBM probe
+ * A measurement aid with no counterpart in the reference: one streaming pass
+ * over `bytes` (a multiple of 16, buffers 16-byte aligned) with 16-byte lanes
+ * and a grid-stride loop over `grid` workgroups of 256 threads, to measure
+ * what this device's HBM delivers for each kind of traffic the kernels move:
+ * MDT_PROBE_READ reads src (dst unused), MDT_PROBE_WRITE writes dst with
+ * streaming stores (src unused), MDT_PROBE_COPY copies src to dst. */
+#define MDT_PROBE_READ 0
+#define MDT_PROBE_WRITE 1
+#define MDT_PROBE_COPY 2
+int mdt_hbm_probe(int kind, const void *src, void *dst, size_t bytes, int grid, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

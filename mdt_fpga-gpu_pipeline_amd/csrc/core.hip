@@ -219,6 +219,28 @@ __global__ void synth_kernel(mdt_synth_params p, int NN, int N, long total, int8
     synth_one(p, NN, N, i, out);
 }
 
+// ------------------------------------------------------------------ HBM probe
+typedef unsigned probe_u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void probe_read_kernel(const probe_u32x4 *__restrict__ a, unsigned *flag, long n) {
+  const long stride = (long)gridDim.x * 256;
+  unsigned acc = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    const probe_u32x4 v = a[i];
+    acc ^= v[0] ^ v[1] ^ v[2] ^ v[3];
+  }
+  if (acc == 0x9e3779b9u && flag) flag[0] = acc;  // keeps the loads; practically never taken
+}
+__global__ __launch_bounds__(256) void probe_write_kernel(probe_u32x4 *__restrict__ b, long n) {
+  const long stride = (long)gridDim.x * 256;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(probe_u32x4{(unsigned)i, 0u, 0u, 0u}, b + i);
+}
+__global__ __launch_bounds__(256) void probe_copy_kernel(const probe_u32x4 *__restrict__ a, probe_u32x4 *__restrict__ b,
+                                                         long n) {
+  const long stride = (long)gridDim.x * 256;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 }  // namespace mdt
 
 using namespace mdt;
@@ -246,6 +268,44 @@ int mdt_pfb_window_exblackman(int Nchannels, int Ntaps, float *Window, hipStream
   return mdt_pfb_window(Nchannels, Ntaps, Window, MDT_WINDOW_BH4, stream);
 }
 
+// Arrays of up to 128 inputs (the MDT station's 38-40), even NN and N,
+// 16-byte aligned buffers: a workgroup moves 16 channels x SS spectra x all
+// NN inputs (SS*NN <= 256 rows (s, e)).  Each row's 16 channels are one
+// 128-byte piece of FOut; in XIn each channel's SS*NN samples are one run
+// ([c][s][e]), written as 16-byte chunks.  The 64 x 32 tile kernel above
+// leaves 3/4 of its threads idle on the second input tile at 40 inputs and
+// writes 256- and 64-byte runs.
+constexpr int RR_C = 16, RR_ROWS = 256;
+__global__ __launch_bounds__(256) void reorder_rows_kernel(int NN, int N, int Ns, int SS,
+                                                           const unsigned long long *__restrict__ F,
+                                                           unsigned long long *__restrict__ X) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) unsigned long long tile[RR_C][RR_ROWS + 2];
+  const int c0 = blockIdx.x * RR_C;
+  const long s0 = (long)blockIdx.y * SS;
+  const int ss = (int)min((long)SS, (long)Ns - s0);
+  const int R = ss * NN;     // rows (s, e) of this workgroup
+  const long r0 = s0 * NN;   // its first row of FOut
+  const int lc = (threadIdx.x & 7) * 2, lr = threadIdx.x >> 3;
+  const int c = c0 + lc;     // this lane's channel pair (N even: both or neither)
+  for (int r = lr; r < R; r += 32) {
+    if (c < N) {
+      const u64x2 v = *(const u64x2 *)(F + (r0 + r) * N + c);
+      tile[lc][r] = v.x;
+      tile[lc + 1][r] = v.y;
+    }
+  }
+  __syncthreads();
+  const int nck = R / 2;     // 16-byte chunks per channel
+  for (int k = threadIdx.x; k < RR_C * nck; k += 256) {
+    const int ch = k / nck, j = k - ch * nck;
+    if (c0 + ch < N) {
+      const u64x2 v = *(const u64x2 *)&tile[ch][2 * j];
+      __builtin_nontemporal_store(v, (u64x2 *)(X + ((long)(c0 + ch) * Ns + s0) * NN + 2 * j));
+    }
+  }
+}
+
 int mdt_reorder_foutput(int NpolsxNelements, int Nchannels, const mdt_complex *FOutput,
                         mdt_complex *XInput, int Nspectra, hipStream_t stream) {
   MDT_TRY(check_inputs(NpolsxNelements));
@@ -255,6 +315,15 @@ int mdt_reorder_foutput(int NpolsxNelements, int Nchannels, const mdt_complex *F
   MDT_REQUIRE_PTR(XInput);
   if (((uintptr_t)FOutput | (uintptr_t)XInput) & 7)
     return fail(MDT_ERR_MISALIGNED, "complex buffers must be 8-byte aligned");
+  if (NpolsxNelements <= 128 && NpolsxNelements % 2 == 0 && Nchannels % 2 == 0 &&
+      (((uintptr_t)FOutput | (uintptr_t)XInput) & 15) == 0) {
+    const int SS = RR_ROWS / NpolsxNelements;
+    const long nsg = (Nspectra + SS - 1) / SS;
+    dim3 grid2((Nchannels + RR_C - 1) / RR_C, (unsigned)nsg);
+    hipLaunchKernelGGL(reorder_rows_kernel, grid2, dim3(256), 0, stream, NpolsxNelements, Nchannels, Nspectra, SS,
+                       (const unsigned long long *)FOutput, (unsigned long long *)XInput);
+    return check_launch("reorder_rows_kernel");
+  }
   dim3 grid((Nchannels + RT_C - 1) / RT_C, (NpolsxNelements + RT_E - 1) / RT_E, Nspectra);
   hipLaunchKernelGGL(reorder_kernel, grid, dim3(256), 0, stream, NpolsxNelements, Nchannels, Nspectra,
                      (const unsigned long long *)FOutput, (unsigned long long *)XInput);
@@ -295,6 +364,7 @@ int mdt_unpack_u32(const uint32_t *words, float *out, size_t n, hipStream_t stre
   return check_launch("unpack_kernel");
 }
 
+
 int mdt_synth_i8(const mdt_synth_params *p, int NpolsxNelements, int Nchannels, int nframes,
                  int8_t *out, hipStream_t stream) {
   MDT_REQUIRE_PTR(p);
@@ -309,6 +379,35 @@ int mdt_synth_i8(const mdt_synth_params *p, int NpolsxNelements, int Nchannels, 
   hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, *p, NpolsxNelements,
                      Nchannels, total, out);
   return check_launch("synth_kernel");
+}
+
+int mdt_hbm_probe(int kind, const void *src, void *dst, size_t bytes, int grid, hipStream_t stream) {
+  using namespace mdt;
+  if (bytes == 0 || bytes % 16 || grid < 1 || grid > (1 << 20))
+    return fail(MDT_ERR_INVALID_VALUE, "HBM probe: bytes %zu (a positive multiple of 16), grid %d", bytes, grid);
+  if (((uintptr_t)src | (uintptr_t)dst) & 15)
+    return fail(MDT_ERR_MISALIGNED, "HBM probe buffers must be 16-byte aligned");
+  const long n = (long)(bytes / 16);
+  switch (kind) {
+    case MDT_PROBE_READ:
+      MDT_REQUIRE_PTR(src);
+      hipLaunchKernelGGL(probe_read_kernel, dim3(grid), dim3(256), 0, stream, (const probe_u32x4 *)src,
+                         (unsigned *)dst, n);
+      break;
+    case MDT_PROBE_WRITE:
+      MDT_REQUIRE_PTR(dst);
+      hipLaunchKernelGGL(probe_write_kernel, dim3(grid), dim3(256), 0, stream, (probe_u32x4 *)dst, n);
+      break;
+    case MDT_PROBE_COPY:
+      MDT_REQUIRE_PTR(src);
+      MDT_REQUIRE_PTR(dst);
+      hipLaunchKernelGGL(probe_copy_kernel, dim3(grid), dim3(256), 0, stream, (const probe_u32x4 *)src,
+                         (probe_u32x4 *)dst, n);
+      break;
+    default:
+      return fail(MDT_ERR_INVALID_VALUE, "HBM probe kind %d", kind);
+  }
+  return check_launch("HBM probe");
 }
 
 }  // extern "C"

@@ -142,6 +142,7 @@ _SIGS = {
     "mdt_calib_loss_grad": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     "mdt_calib_plan_destroy": [_VP],
     "mdt_calib_loss_grad_noalloc": [_I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
+    "mdt_hbm_probe": [_I, _VP, _VP, _SZ, _I, _VP],
 }
 
 
@@ -785,3 +786,17 @@ def synth_i8(NN, N, nframes, seed=1, sigma=16.0, rho=0.0, max_delay=0, tones=(),
         out = torch.empty((nframes, NN, N, 2), dtype=torch.int8, device="cuda")
     check(lib().mdt_synth_i8(ctypes.byref(p), NN, N, nframes, _ptr(out, torch.int8), _stream()))
     return out
+
+
+PROBE_READ, PROBE_WRITE, PROBE_COPY = 0, 1, 2
+
+
+def hbm_probe(kind, src, dst, grid):
+    """One streaming pass of the library's HBM probe (read src, write dst, or
+    copy src to dst; uint8 CUDA tensors of equal size, multiple of 16 bytes)
+    on the current stream: a measurement aid for the kernels' rooflines."""
+    t = src if src is not None else dst
+    nbytes = t.numel() * t.element_size()
+    sp = ctypes.c_void_p(src.data_ptr()) if src is not None else ctypes.c_void_p(0)
+    dp = ctypes.c_void_p(dst.data_ptr()) if dst is not None else ctypes.c_void_p(0)
+    check(lib().mdt_hbm_probe(kind, sp, dp, nbytes, grid, _stream()))

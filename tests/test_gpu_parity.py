@@ -140,7 +140,8 @@ def test_fft_tone_single_bin():
 
 
 # ----------------------------------------------------------------- a4 corner turn
-@pytest.mark.parametrize("NN,N,Ns", [(3, 32, 2), (38, 1024, 7), (40, 2048, 3), (70, 100, 5), (40, 100, 4), (64, 96, 3)])
+@pytest.mark.parametrize("NN,N,Ns", [(3, 32, 2), (38, 1024, 7), (40, 2048, 3), (70, 100, 5), (40, 100, 4), (64, 96, 3),
+                                      (128, 64, 5), (40, 101, 3), (130, 64, 3), (4, 16384, 300)])
 def test_reorder_bit_exact(NN, N, Ns):
     print(f"reorder NN={NN} N={N} Ns={Ns}", flush=True)
     rng = _rng(NN * N)
@@ -276,3 +277,28 @@ def test_unpack_u32_exhaustive():
         assert np.array_equal(f.view(np.uint32), ref.view(np.uint32)), hex(start)
     w = torch.tensor([-2**31], dtype=torch.int32, device="cuda")
     assert _host(mdtfx.unpack_u32(w))[0] == 4096.0
+
+
+# ----------------------------------------------------------------- HBM probe (measurement aid)
+def test_hbm_probe_copy_write_read():
+    """the probe kernels behind bench.py's measured HBM ceilings move the bytes
+    they are said to move: copy is exact, write stores its pattern, read and
+    bad arguments behave"""
+    print("hbm probe", flush=True)
+    n = (1 << 20) + 48
+    src = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.zeros_like(src)
+    mdtfx.hbm_probe(mdtfx.PROBE_COPY, src, dst, 7)
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    mdtfx.hbm_probe(mdtfx.PROBE_WRITE, None, dst, 5)
+    torch.cuda.synchronize()
+    w = dst.view(torch.int32).view(-1, 4).cpu().numpy()
+    assert np.array_equal(w[:, 0], np.arange(w.shape[0], dtype=np.int32)) and not w[:, 1:].any()
+    mdtfx.hbm_probe(mdtfx.PROBE_READ, src, None, 3)
+    torch.cuda.synchronize()
+    with pytest.raises(mdtfx.MdtError):
+        mdtfx.hbm_probe(mdtfx.PROBE_COPY, src[1:17], dst[:16], 1)  # misaligned source
+    with pytest.raises(mdtfx.MdtError):
+        mdtfx.hbm_probe(7, src, dst, 1)
+
