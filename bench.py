@@ -126,13 +126,14 @@ def fp8_leg(np, torch, mdtfx, steps, warmup, stream, err_channels, with_err):
     return out
 
 
-def stream_leg(torch, mdtfx, x8, dumps):
+def stream_leg(torch, mdtfx, x8, dumps, err_channels=0):
     """The same scaled-array dumps through the streaming correlator: int8 stacks from
     the two pinned host slots over PCIe (the history carried on the device),
     F+X, visibilities back to pinned host memory, copies and compute on
     separate streams.  Reports the host-inclusive rate beside the
     device-resident value.  The host stacks are filled once; every dump
-    copies a full stack."""
+    copies a full stack.  With err_channels: the last dump's visibilities
+    against the FP64 oracle on its window (the carried history, the stack)."""
     plan = mdtfx.FxPlan(NN, N, T, NS, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP16_SPLIT)
     st = mdtfx.FxStream(plan)
     for slot in range(2):
@@ -155,12 +156,29 @@ def stream_leg(torch, mdtfx, x8, dumps):
     t0 = time.perf_counter()
     times = run(dumps)
     wall = (time.perf_counter() - t0) * 1e3 / dumps
+    vis_last = st.vis((dumps - 1) % 2).copy()
     st.destroy()
     mean = {k: sum(t[k] for t in times) / len(times) for k in times[0]}
     stack_bytes = NS * NN * N * 2
-    return {"value": NS * N / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_dump": wall,
-            "dumps": dumps, "phase_ms": mean, "h2d_GB/s": stack_bytes / (mean["h2d_ms"] * 1e-3) / 1e9,
-            "stack_bytes": stack_bytes}
+    out = {"value": NS * N / (wall * 1e-3) / 1e6, "unit": "Msamples/s/input", "ms_per_dump": wall,
+           "dumps": dumps, "phase_ms": mean, "h2d_GB/s": stack_bytes / (mean["h2d_ms"] * 1e-3) / 1e9,
+           "stack_bytes": stack_bytes}
+    if err_channels:
+        import numpy as np
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        # every dump after the first carries the stack's last T-1 spectra
+        xh = x8.cpu().numpy()
+        window = np.concatenate([xh[-(T - 1):], xh[T - 1:]])
+        chans = np.unique(np.linspace(0, N - 1, err_channels).astype(int))
+        t0 = time.perf_counter()
+        v64 = orc.fx_chain_f64(plan.window().cpu().numpy().astype(np.float64), window, NN, N, T, NS, chans)
+        out["max_rel_err_vs_fp64"] = {
+            "last_dump_fp16_split": _autonorm_err(np, orc, vis_last[chans], v64, NN),
+            "channels_checked": len(chans), "oracle_seconds": round(time.perf_counter() - t0, 1),
+            "definition": "max over (channel, a>=b) of |V - V64| / sqrt(V64_aa V64_bb)"}
+    return out
 
 
 def station_leg(np, torch, mdtfx, steps, warmup, err_channels=0):
@@ -371,7 +389,7 @@ def calib_leg(np, torch, mdtfx, steps, warmup, with_err):
     return out
 
 
-def packet_stream_leg(np, torch, mdtfx, dumps):
+def packet_stream_leg(np, torch, mdtfx, dumps, err_channels=0):
     """The client's loop on its own wire data at the shipped station's wire
     shape (one station: 38 inputs from 4 SNAPs, 2048 channels, 4-tap PFB,
     4096 spectra per dump; 32 samples per element per packet): each dump's
@@ -379,7 +397,9 @@ def packet_stream_leg(np, torch, mdtfx, dumps):
     memory over PCIe, decoded on the device into the float F input behind
     the history, F-engine on the float input, FP32 X-engine, visibilities
     back.  Packet payloads are built once from int8 samples; each dump only
-    renumbers the packet counters."""
+    renumbers the packet counters.  With err_channels: the last dump's
+    visibilities against the FP64 oracle on its window (the Fix_32_19 words
+    decode to the int8 samples exactly)."""
     NN6, N6, T6, NS6, S = 38, 2048, 4, 4096, 32
     words = 4 + 2 * S * 10
     log(f"bench: packet stream leg NN={NN6} N={N6} T={T6} Ns={NS6} S={S}")
@@ -393,7 +413,9 @@ def packet_stream_leg(np, torch, mdtfx, dumps):
         rows = pk[sn * ncnt:(sn + 1) * ncnt]
         rows[:, 1] = sn
         rows[:, 4:4 + 2 * S * nin] = ((v.astype(np.int32) << 19).view(np.uint32))
-    del X, x8
+    del X
+    if not err_channels:
+        del x8
     plan = mdtfx.FxPlan(NN6, N6, T6, NS6, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP32)
     ps = mdtfx.PacketStream(plan, words, S, len(pk))
     for slot in range(2):
@@ -418,6 +440,7 @@ def packet_stream_leg(np, torch, mdtfx, dumps):
     t0 = time.perf_counter()
     times = run(dumps, 2)
     wall = (time.perf_counter() - t0) * 1e3 / dumps
+    vis_last = ps.vis((2 + dumps - 1) % 2).copy()
     ps.destroy()
     mean = {k: sum(t[0][k] for t in times) / len(times) for k in times[0][0]}
     pkt_bytes = 4 * ncnt * words * 4
@@ -427,6 +450,20 @@ def packet_stream_leg(np, torch, mdtfx, dumps):
            "dumps": dumps, "phase_ms": mean, "packet_bytes": pkt_bytes,
            "h2d_GB/s": pkt_bytes / (mean["h2d_ms"] * 1e-3) / 1e9,
            "counts_last_dump": times[-1][1]}
+    if err_channels:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+
+        # dumps after the first carry the previous payload's last T-1 spectra
+        window = np.concatenate([x8[-(T6 - 1):], x8])
+        chans = np.unique(np.linspace(0, N6 - 1, err_channels).astype(int))
+        t0 = time.perf_counter()
+        v64 = orc.fx_chain_f64(plan.window().cpu().numpy().astype(np.float64), window, NN6, N6, T6, NS6, chans)
+        out["max_rel_err_vs_fp64"] = {
+            "last_dump_fp32": _autonorm_err(np, orc, vis_last[chans], v64, NN6),
+            "channels_checked": len(chans), "oracle_seconds": round(time.perf_counter() - t0, 1),
+            "definition": "max over (channel, a>=b) of |V - V64| / sqrt(V64_aa V64_bb)"}
+        del x8
     del plan
     torch.cuda.empty_cache()
     return out
@@ -667,7 +704,8 @@ def main():
     }
     if not args.no_stream:
         log("bench: streaming leg (host stacks over PCIe)")
-        out["stream_host_inclusive"] = stream_leg(torch, mdtfx, x8, max(4, min(args.steps, 8)))
+        out["stream_host_inclusive"] = stream_leg(torch, mdtfx, x8, max(4, min(args.steps, 8)),
+                                                  0 if args.no_cpu else 8)
     if not args.no_cpu:
         log("bench: CPU baseline + FP64 error legs")
         cpu, err = cpu_baseline_and_error(np, torch, plan, x8, vis16.cpu().numpy(), vis32.cpu().numpy(),
@@ -685,7 +723,7 @@ def main():
     if not args.no_calib:
         out["calibration"] = calib_leg(np, torch, mdtfx, min(args.steps, 10), min(args.warmup, 2), not args.no_cpu)
     if not args.no_stream:
-        out["packet_stream_host_inclusive"] = packet_stream_leg(np, torch, mdtfx, 4)
+        out["packet_stream_host_inclusive"] = packet_stream_leg(np, torch, mdtfx, 4, 0 if args.no_cpu else 16)
     if not args.no_hbm_probe:
         probe = hbm_probe_leg(torch, mdtfx)
         # the dominant kernel against what this device measured: its reads at
