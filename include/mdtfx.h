@@ -263,7 +263,7 @@ int mdt_fx_get_window(mdt_fx_plan plan, float *Window);
  * MDT_XPREC_FP32; for MDT_XPREC_FP16_SPLIT each 8-byte sample holds FP16
  * (hi_re, hi_im, lo_re, lo_im) of x*s, s the plan's power-of-two scale,
  * hi = fp16(x*s), lo = fp16(x*s - hi).  MDT_XPREC_FP8 (Nchannels in
- * [512, 4096], Ntaps 4 or 8) has two formats by channel count:
+ * [512, 4096]) has two formats by channel count:
  *   512 and 1024 channels: each sample is 2 bytes, e4m3 (re, im) of x / 2^eb
  *     (nearest-even), eb one exponent per (input, block of 16 spectra) shared
  *     by all channels, the smallest with max |re|, |im| over the block <=

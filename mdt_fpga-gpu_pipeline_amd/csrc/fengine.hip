@@ -231,24 +231,30 @@ __device__ __forceinline__ int stage_idx(int row, int col) {
     return row * C::ROWW + col;
 }
 
+// Wt[((m*G4 + i/4)*TPB + t)*4 + i%4] = W[i*N + t + m*TPB], G4 = ceil(T/4)
+// groups of four taps, taps i >= T zero
 __global__ void arrange_taps_kernel(int N, int T, const float *__restrict__ W, float *__restrict__ Wt) {
+  const int G4 = (T + 3) / 4;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= N * T) return;
-  const int TPB = N / fft_points(N), T4 = T / 4;
+  if (idx >= N * 4 * G4) return;
+  const int TPB = N / fft_points(N);
   const int ii = idx & 3;
   int r = idx >> 2;
   const int t = r % TPB;
   r /= TPB;
-  const int i4 = r % T4, m = r / T4;
-  Wt[idx] = W[(long)(4 * i4 + ii) * N + t + m * TPB];
+  const int i4 = r % G4, m = r / G4;
+  const int i = 4 * i4 + ii;
+  Wt[idx] = i < T ? W[(long)i * N + t + m * TPB] : 0.0f;
 }
 
-bool persistent_shape(int N, int T) { return N >= 512 && N <= 4096 && (T == 4 || T == 8); }
+bool persistent_shape(int N, int T) { return N >= 512 && N <= 4096 && T >= 4 && T <= MDT_MAX_TAPS; }
+int arranged_taps_floats(int N, int T) { return N * 4 * ((T + 3) / 4); }
 bool fp8_block_shape(int N) { return N == 512 || N == 1024; }
 
 int arrange_taps(int N, int T, const float *W, float *Wt, hipStream_t st) {
   if (!persistent_shape(N, T)) return fail(MDT_ERR_UNSUPPORTED, "no persistent F-engine for N=%d T=%d", N, T);
-  hipLaunchKernelGGL(arrange_taps_kernel, dim3((N * T + 255) / 256), dim3(256), 0, st, N, T, W, Wt);
+  hipLaunchKernelGGL(arrange_taps_kernel, dim3((arranged_taps_floats(N, T) + 255) / 256), dim3(256), 0, st, N, T,
+                     W, Wt);
   return check_launch("arrange_taps_kernel");
 }
 
@@ -268,6 +274,8 @@ typedef unsigned v4u __attribute__((ext_vector_type(4)));
 // waves per SIMD the register budget targets: 4 (128 VGPRs) for the plain
 // outputs; the gain stage, the FP8 power sums at N >= 2048 and 16 points per
 // thread (N = 1024: 4 spectra x 16 points are 128 registers) need more room
+__device__ __forceinline__ int arranged_taps_floats_dev(int N, int T) { return N * 4 * ((T + 3) / 4); }
+
 template <int N, int SPLIT, bool GAIN>
 constexpr int fe_waves_per_simd() {
   return (GAIN || fft_points(N) == 16) ? 2 : ((SPLIT == 2 && N >= 2048) ? 3 : 4);
@@ -315,7 +323,7 @@ struct FeDma {
   static constexpr int MR = fits(4) ? 4 : (fits(2) ? 2 : 1);
   static constexpr int XB = xb(MR);
   static constexpr bool ENABLED =
-      fits(MR) && (PAIR ? (T == 8 && !GAIN && SPLIT <= 1 && C::Q == 2 && FRB >= 3 * C::FFT_LDS &&
+      T > 0 && fits(MR) && (PAIR ? (T == 8 && !GAIN && SPLIT <= 1 && C::Q == 2 && FRB >= 3 * C::FFT_LDS &&
                            C::FFT_LDS >= ROWS && N / ipow_c(C::P, L) == 1)
                         : ((N == 1024 || (N == 2048 && SPLIT != 0)) && FRB >= C::Q * C::FFT_LDS));
 };
@@ -406,7 +414,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   constexpr bool DMA = D::ENABLED;                       // frames prefetched into LDS
   constexpr bool PAIR = DMA && D::PAIR;                  // two transforms per sub-block
   constexpr bool TAPS_LDS =
-      DMA ? D::TAPS : (T * N * 4 <= 32768 && (C::TWN + C::BUF) * 8 + T * N * 4 <= 150 * 1024);
+      DMA ? D::TAPS : (T > 0 && T * N * 4 <= 32768 && (C::TWN + C::BUF) * 8 + T * N * 4 <= 150 * 1024);
   constexpr int NF = SPAN + T - 1;                       // input frames per item
   constexpr int TWN = DMA ? D::TWN : C::TWN;
   constexpr int MR = DMA ? D::MR : C::MR;                // staging rounds per barrier pair
@@ -461,8 +469,9 @@ fengine_persistent_kernel(FArgs a, long nitems) {
   const long G = gridDim.x;
   const long b = blockIdx.x;
   const long spec = (long)a.NN * N;
-  const __amdgpu_buffer_rsrc_t rtap =
-      __builtin_amdgcn_make_buffer_rsrc((void *)a.Wt, (short)0, T * N * 4, kBufferWord3);
+  // the arranged window: T x N floats, or its zero-padded groups of four
+  const __amdgpu_buffer_rsrc_t rtap = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)a.Wt, (short)0, (T > 0 ? T * N : arranged_taps_floats_dev(N, a.T)) * 4, kBufferWord3);
   float2 *fbuf = (DMA ? frb : buf) + q * C::FFT_LDS;  // this sub-block's exchange buffer
   // XCD-aware: workgroup b runs on XCD b % 8; consecutive logical items (the
   // spectrum blocks of one input: shared FIR frames, adjacent output pieces)
@@ -527,6 +536,42 @@ fengine_persistent_kernel(FArgs a, long nitems) {
     const PassTw<P> tws{twl + twoff};
     int tl = opaque(t);
     float2 acc[SB][P];
+    if constexpr (T == 0) {
+      // any tap count: the arranged window is zero-padded to a multiple of 4
+      // taps and the FIR runs over groups of four, each with its SB + 3
+      // frames.  A zero tap adds +0 or -0 to a sum that is never -0, so the
+      // sums are those of the T-tap FIR in tap order, bit for bit.
+      const int G4 = (a.T + 3) >> 2;
+#pragma unroll
+      for (int m = 0; m < P; m++) {
+        float2 y[SB];
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) y[sl] = make_float2(0.f, 0.f);
+        for (int g = 0; g < G4; g++) {
+          const v4u qv = __builtin_amdgcn_raw_buffer_load_b128(rtap, 16 * tl, 16 * ((m * G4 + g) * TPB), 0);
+          const float w[4] = {__uint_as_float(qv[0]), __uint_as_float(qv[1]), __uint_as_float(qv[2]),
+                              __uint_as_float(qv[3])};
+          float2 x[SB + 3];
+#pragma unroll
+          for (int f = 0; f < SB + 3; f++) {
+            const int ff = SB * q + 4 * g + f;
+            const int fr = ff < fmax ? ff : fmax;
+            const unsigned short u =
+                __builtin_amdgcn_raw_buffer_load_b16(rin, 2 * (tl + m * TPB), (int)(2 * (long)fr * spec), 0);
+            x[f] = make_float2((float)(signed char)(u & 0xff), (float)(signed char)(u >> 8));
+          }
+#pragma unroll
+          for (int sl = 0; sl < SB; sl++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              y[sl].x = __builtin_fmaf(w[i], x[sl + i].x, y[sl].x);
+              y[sl].y = __builtin_fmaf(w[i], x[sl + i].y, y[sl].y);
+            }
+        }
+#pragma unroll
+        for (int sl = 0; sl < SB; sl++) acc[sl][m] = y[sl];
+      }
+    } else {
 #pragma unroll
     for (int m = 0; m < P; m++) {
       float w[T];
@@ -567,6 +612,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         }
         acc[sl][m] = y;
       }
+    }
     }
     // transforms, one at a time, all SB of them: inside `if (sl < scount)`
     // the FIR sums of the later spectra would be sunk into those branches,
@@ -933,10 +979,11 @@ static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   if constexpr (N >= 512 && N <= 4096) {
     if (!fin && a.T == 4) return launch_persistent<N, 4>(a, st);
     if (!fin && a.T == 8) return launch_persistent<N, 8>(a, st);
+    if (!fin && persistent_shape(N, a.T)) return launch_persistent<N, 0>(a, st);
   }
   if (a.split >= 2)
     return fail(MDT_ERR_UNSUPPORTED,
-                "FP8 X-engine input needs int8 samples, Nchannels in [512, 4096] and Ntaps 4 or 8");
+                "FP8 X-engine input needs int8 samples and Nchannels in [512, 4096]");
   const long nsb = (a.Ns + C::SB - 1) / C::SB;
   const long neg = (a.NN + C::EB - 1) / C::EB;
   const long total = nsb * neg;

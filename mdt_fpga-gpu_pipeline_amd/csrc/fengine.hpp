@@ -72,13 +72,16 @@ __device__ __forceinline__ float2 split_sample(float2 v, float sc) {
 
 // int8 input (a.in) or float2 input (a.fin, the generic kernel only)
 int launch_fengine(const FArgs &a, int N, hipStream_t st);
-// shapes served by the persistent kernel (N in [512, 4096], T in {4, 8})
+// shapes served by the persistent kernel (N in [512, 4096], any tap count;
+// 4 and 8 taps with their own code, the others in groups of four)
 bool persistent_shape(int N, int T);
+// floats of the arranged window (taps zero-padded to a multiple of 4)
+int arranged_taps_floats(int N, int T);
 // FP8 sample formats by shape: e4m3 with block scales where an F-engine item
 // spans at least 16 spectra (N = 512, 1024), per-channel FP16 + power above
 bool fp8_block_shape(int N);
-// the window [T][N] re-arranged for the persistent kernel: Wt[((m*(T/4) + i/4)*(N/8) + t)*4 + i%4]
-// = W[i*N + t + m*N/8]
+// the window [T][N] re-arranged for the persistent kernel: Wt[((m*G4 + i/4)*TPB + t)*4 + i%4]
+// = W[i*N + t + m*TPB], G4 = ceil(T/4), TPB = N / (points per thread), taps i >= T zero
 int arrange_taps(int N, int T, const float *W, float *Wt, hipStream_t st);
 
 }  // namespace mdt

@@ -148,8 +148,8 @@ static int check_xprec(int xprec, int N, int T) {
   if (xprec != MDT_XPREC_FP32 && xprec != MDT_XPREC_FP16_SPLIT && xprec != MDT_XPREC_FP8)
     return fail(MDT_ERR_INVALID_VALUE, "unknown X-engine precision %d", xprec);
   // the FP8 sample format and channel powers come from the persistent F-engine
-  if (xprec == MDT_XPREC_FP8 && (N < 512 || N > 4096 || (T != 4 && T != 8)))
-    return fail(MDT_ERR_UNSUPPORTED, "FP8 path needs Nchannels in [512, 4096] and Ntaps 4 or 8");
+  if (xprec == MDT_XPREC_FP8 && !persistent_shape(N, T))
+    return fail(MDT_ERR_UNSUPPORTED, "FP8 path needs Nchannels in [512, 4096]");
   return MDT_SUCCESS;
 }
 
@@ -197,7 +197,7 @@ int mdt_fx_plan_create(mdt_fx_plan *plan, int NpolsxNelements, int Nchannels, in
     st = check_launch("zero_kernel");
   }
   if (st == MDT_SUCCESS && persistent_shape(Nchannels, Ntaps)) {
-    if (hipMalloc(&p->Wt, sizeof(float) * (size_t)Ntaps * Nchannels) != hipSuccess)
+    if (hipMalloc(&p->Wt, sizeof(float) * (size_t)arranged_taps_floats(Nchannels, Ntaps)) != hipSuccess)
       st = fail(MDT_ERR_ALLOC, "arranged window");
     else
       st = arrange_taps(Nchannels, Ntaps, p->W, p->Wt, 0);

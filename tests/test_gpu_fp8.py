@@ -79,6 +79,7 @@ BLOCK_CASES = [
     # past the dump); 200 inputs (one triangle workgroup), 48 spectra
     (300, 512, 4, 70, 100, 40.0, 4.0, None),
     (200, 1024, 4, 48, 500, 40.0, 4.0, None),
+    (64, 1024, 12, 64, 321, 40.0, 4.0, None),   # 12 taps: the FIR in groups of four
 ]
 
 
@@ -207,6 +208,7 @@ CHAN_CASES = [
     # NN, N, T, Ns, tone bin, tone amplitude, sigma
     (64, 2048, 4, 64, 600, 40.0, 4.0),
     (40, 4096, 8, 40, 1234, 40.0, 4.0),
+    (40, 2048, 6, 40, 700, 40.0, 4.0),   # 6 taps, zero-padded to 8
 ]
 
 
@@ -280,9 +282,10 @@ def test_channel_determinism(chancase):
 
 
 def test_fp8_shape_limits():
-    for N, T in ((256, 4), (8192, 4), (1024, 5)):
+    for N, T in ((256, 4), (8192, 4), (1024, 3)):
         with pytest.raises(mdtfx.MdtError):
             mdtfx.FxPlan(8, N, T, 16, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP8)
+    mdtfx.FxPlan(8, 1024, 5, 16, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP8)  # any tap count in [4, 128]
     plan = mdtfx.FxPlan(8, 256, 4, 16, mdtfx.WINDOW_BH4, mdtfx.XPREC_FP32)
     with pytest.raises(mdtfx.MdtError):
         plan.set_precision(mdtfx.XPREC_FP8)

@@ -27,7 +27,9 @@ def _tri_autonorm_err(vis, v64, NN):
 
 @pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (3, 32, 4, 11), (5, 64, 8, 17),
                                        (4, 2048, 4, 9), (3, 4096, 8, 16), (37, 4096, 8, 69), (3, 8192, 4, 5),
-                                       (3, 16384, 4, 3), (6, 256, 5, 12)])
+                                       (3, 16384, 4, 3), (6, 256, 5, 12),
+                                       (6, 1024, 16, 40), (5, 2048, 5, 33), (3, 4096, 12, 20), (300, 512, 7, 70),
+                                       (4, 1024, 128, 9)])
 @pytest.mark.parametrize("layout", [mdtfx.XIN_ROWS, mdtfx.XIN_PAIRS, mdtfx.XIN_BLOCKS])
 def test_fused_fengine_equals_dropin_composition(NN, N, T, Ns, layout):
     """fused PFB+FFT+corner turn == pps_batch_i8 -> fft -> reorder, bit for
@@ -231,7 +233,8 @@ def test_synth_beyond_32bit_grid():
 
 
 @pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (256, 4096, 8, 64), (130, 512, 8, 100), (3, 64, 4, 9),
-                                       (300, 256, 4, 40)])  # > 256 inputs: 128-input tiles, ragged
+                                       (300, 256, 4, 40),  # > 256 inputs: 128-input tiles, ragged
+                                       (40, 1024, 16, 64), (64, 2048, 6, 48)])  # tap groups of four
 def test_fp16_split_vs_fp64(NN, N, T, Ns):
     """FP16 hi+lo split X-engine within the 1e-5 autocorrelation-normalised bar
     (the 256-input scaled-array shape at reduced integration length)."""
