@@ -219,40 +219,54 @@ __global__ __launch_bounds__(256) void xeng_f32_kernel(XArgs g) {
 
 // Hermitian products of small arrays (NN <= 48, the MDT station's 40
 // inputs): the drop-in CGEMM with the reference's X-input layout (NN even,
-// one sample's NN rows contiguous, ld == NN: a chunk of 64 samples is one
-// contiguous span of 64·NN complex values) and the native triangle in any
-// X-input layout (a chunk is 64 samples of each row), both loaded as 16-byte
-// pieces into registers while the previous chunk is multiplied.  16 x 16
-// blocks on v_mfma_f32_16x16x4f32 pad 40 inputs to 48 rather than 64, and
-// two accumulator sets per block (alternate sample pairs) give each wave four
-// to eight independent MFMA chains: with one set the dependent MFMAs'
-// latency set the time (2.30 against 1.21 ms at the station's shape).  One
-// workgroup per batch entry; the nb(nb+1)/2 lower-triangle blocks are dealt
-// to the four waves, rotated per workgroup so that the waves with two blocks
-// fall on different SIMDs.  Operand layout: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16], K = 4 being two complex samples (k = 2t + re/im); D
-// register r of lane l is D[4(l/16) + r][l%16].  Products and epilogues as
-// in xeng_f32_kernel.
+// one sample's NN rows contiguous, ld == NN: a chunk of 32 samples is one
+// contiguous span of 32·NN complex values) and the native triangle in any
+// X-input layout (a chunk is 32 samples of each row).  16 x 16 blocks on
+// v_mfma_f32_16x16x4f32 pad 40 inputs to 48 rather than 64.  Operand layout:
+// lane l supplies A[l%16][l/16] and B[l/16][l%16], K = 4 being two complex
+// samples (k = 2t + re/im); D register r of lane l is D[4(l/16) + r][l%16].
+// Products and epilogues as in xeng_f32_kernel.
+//
+// One 128-thread workgroup per batch entry.  The nb(nb+1)/2 blocks go to
+// the two waves with fixed roles, split once at the top so that each wave's
+// loop carries no per-block branch (a per-block `if` put every MFMA pair
+// behind its own exec mask, each waiting for its LDS reads: 1.16 against
+// 0.90 ms at the station's shape).  In the full form, with (I, J) the A
+// and B block rows and I <= J: role 0 = (0,0), (0,1), (0,2), role 1 =
+// (1,1), (1,2), (2,2); the triangle uses the transposed blocks.  Each block
+// has two accumulator sets (alternate sample pairs, summed at the end), so a
+// wave runs six to twelve independent MFMA chains.  The chunk sits in LDS as
+// complex [sample][row] with rows NN..47 zero (a 16-byte piece of the
+// drop-in layout is one 16-byte LDS store; the native layout's pieces are
+// two samples of one row, written as two 8-byte stores, with an odd row
+// pitch so that the 16 lanes of a row hit distinct banks); each wave reads
+// its three 16-row operand groups per sample pair one step ahead of the
+// MFMAs that use them.  The next chunk is loaded into registers while the
+// current one is multiplied.  128 VGPRs + 48 accumulation registers: four
+// waves per SIMD.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int XS = 16;                       // block edge
 constexpr int XS_MAXN = 48;                  // largest NN
-constexpr int XS_KC = 64;                    // samples per chunk
-constexpr int XS_RL = XS_MAXN + 1;           // LDS plane row length (floats)
-constexpr int XS_PLANE = 2 * XS_RL;          // floats per sample: re row + im row
-constexpr int XS_V4 = XS_KC * XS_MAXN / 2 / 256;  // 16-byte pieces per thread per chunk (6)
+constexpr int XS_KC = 32;                    // samples per chunk
+constexpr int XS_NT = 128;                   // threads per workgroup
+constexpr int XS_V4 = XS_KC * XS_MAXN / 2 / XS_NT;  // 16-byte pieces per thread per chunk (6)
+template <int LAYOUT>
+struct XsRow {
+  static constexpr int RL = LAYOUT == XL_KM ? XS_MAXN : XS_MAXN + 1;  // LDS row pitch (complex values)
+};
 
-// al16 (uniform): the operand base is 16-byte aligned, so a piece is one
-// 16-byte load; otherwise two 8-byte loads (table entries are not checked on
-// the host)
-__device__ __forceinline__ void xs_fetch(float4 (&v)[XS_V4], const float2 *base, int NN, int s0, int K,
-                                         bool al16) {
+// AL16: the operand base is 16-byte aligned, so a piece is one 16-byte
+// load; otherwise two 8-byte loads (table entries are checked per workgroup,
+// not on the host)
+template <bool AL16>
+__device__ __forceinline__ void xs_fetch(float4 (&v)[XS_V4], const float2 *base, int NN, int s0, int K) {
   const long nval = (long)(K - s0 < XS_KC ? K - s0 : XS_KC) * NN / 2;  // valid pieces (NN even)
   const float2 *src = base + (long)s0 * NN;
 #pragma unroll
   for (int i = 0; i < XS_V4; i++) {
-    const int q = threadIdx.x + i * 256;
+    const int q = threadIdx.x + i * XS_NT;
     if (q < nval) {
-      if (al16) {
+      if constexpr (AL16) {
         v[i] = ((const float4 *)src)[q];
       } else {
         const float2 x0 = src[2 * q], x1 = src[2 * q + 1];
@@ -263,18 +277,31 @@ __device__ __forceinline__ void xs_fetch(float4 (&v)[XS_V4], const float2 *base,
     }
   }
 }
+// pieces beyond the chunk's valid samples were zeroed by the fetch, so every
+// sample of the chunk is written (rows NN..47 stay zero from the start)
+__device__ __forceinline__ void xs_stage(const float4 (&v)[XS_V4], float2 *lds, int NN) {
+  const int npiece = XS_KC * NN / 2;
+#pragma unroll
+  for (int i = 0; i < XS_V4; i++) {
+    const int q = threadIdx.x + i * XS_NT;
+    if (q < npiece) {
+      const int e = 2 * q, s = e / NN, r = e - s * NN;  // r, r + 1 in the same sample (NN even)
+      *(float4 *)(lds + s * XsRow<XL_KM>::RL + r) = v[i];
+    }
+  }
+}
 
 // The native X-input layouts (element (row e, sample s) at
 // e*ld + xin_spec(lgB, bp, s); ld, bp and the block length even, so a pair
 // of samples (s even) is one aligned 16-byte piece inside one block): piece q
-// is samples 2(q%32), 2(q%32)+1 of row q/32; samples past K are zeroed (the
+// is samples 2(q%16), 2(q%16)+1 of row q/16; samples past K are zeroed (the
 // pair's second sample lies in the row's or the block's padding then)
 __device__ __forceinline__ void xs_fetch_mk(float4 (&v)[XS_V4], const float2 *base, long ld, int lgB, long bp,
                                             int NN, int s0, int K) {
 #pragma unroll
   for (int i = 0; i < XS_V4; i++) {
-    const int q = threadIdx.x + i * 256;
-    const int row = q >> 5, s = s0 + 2 * (q & 31);
+    const int q = threadIdx.x + i * XS_NT;
+    const int row = q >> 4, s = s0 + 2 * (q & 15);
     if (row < NN && s < K) {
       float4 x = *(const float4 *)(base + row * ld + xin_spec(lgB, bp, s));
       if (s + 1 >= K) x.z = x.w = 0.f;
@@ -284,75 +311,42 @@ __device__ __forceinline__ void xs_fetch_mk(float4 (&v)[XS_V4], const float2 *ba
     }
   }
 }
-__device__ __forceinline__ void xs_stage_mk(const float4 (&v)[XS_V4], float *lds, int NN) {
+__device__ __forceinline__ void xs_stage_mk(const float4 (&v)[XS_V4], float2 *lds, int NN) {
+  constexpr int RL = XsRow<XL_MK>::RL;
 #pragma unroll
   for (int i = 0; i < XS_V4; i++) {
-    const int q = threadIdx.x + i * 256;
-    const int r = q >> 5, s = 2 * (q & 31);
+    const int q = threadIdx.x + i * XS_NT;
+    const int r = q >> 4, s = 2 * (q & 15);
     if (r < NN) {
-      float *p = lds + s * XS_PLANE + r;
-      p[0] = v[i].x;
-      p[XS_RL] = v[i].y;
-      p[XS_PLANE] = v[i].z;
-      p[XS_PLANE + XS_RL] = v[i].w;
+      lds[s * RL + r] = make_float2(v[i].x, v[i].y);
+      lds[(s + 1) * RL + r] = make_float2(v[i].z, v[i].w);
     }
   }
 }
 
-// pieces beyond the chunk's valid samples were zeroed by the fetch, so every
-// sample row of the chunk is written (rows NN..47 stay zero from the start)
-__device__ __forceinline__ void xs_stage(const float4 (&v)[XS_V4], float *lds, int NN) {
-  const int npiece = XS_KC * NN / 2;
-#pragma unroll
-  for (int i = 0; i < XS_V4; i++) {
-    const int q = threadIdx.x + i * 256;
-    if (q < npiece) {
-      const int e = 2 * q, s = e / NN, r = e - s * NN;  // r, r + 1 in the same sample (NN even)
-      float *p = lds + s * XS_PLANE + r;
-      p[0] = v[i].x;
-      p[XS_RL] = v[i].y;
-      p[1] = v[i].z;
-      p[XS_RL + 1] = v[i].w;
-    }
-  }
+// blocks of one role: slot q -> (I, J), the A and B operand block rows
+template <int OUT, int ROLE>
+__device__ __forceinline__ constexpr int xs_blk_I(int q) {
+  const int lo = ROLE == 0 ? 0 : (q == 2 ? 2 : 1), hi = ROLE == 0 ? q : (q == 0 ? 1 : 2);
+  return OUT == XO_FULL ? lo : hi;
+}
+template <int OUT, int ROLE>
+__device__ __forceinline__ constexpr int xs_blk_J(int q) {
+  const int lo = ROLE == 0 ? 0 : (q == 2 ? 2 : 1), hi = ROLE == 0 ? q : (q == 0 ? 1 : 2);
+  return OUT == XO_FULL ? hi : lo;
 }
 
-template <int LAYOUT, int OUT>
-__global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
-  __shared__ float lds[XS_KC * XS_PLANE];
-
-  const long nwg = gridDim.x;
-  const long w = blockIdx.x;
-  const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
-  const int batch = (int)logical;
-  const float2 *base;
-  if constexpr (OUT == XO_TRI)
-    base = g.Abase + xin_batch(g, batch);
-  else
-    base = g.Btab ? g.Btab[batch] : g.Bbase + (long)batch * g.Bbatch;
+template <int LAYOUT, int OUT, int ROLE, int NB, bool AL16>
+__device__ __forceinline__ void xs_role(const XArgs &g, int batch, const float2 *base, float2 *lds) {
+  constexpr int RL = XsRow<LAYOUT>::RL;
+  constexpr int NQ = ROLE == 0 ? NB : (NB == 3 ? 3 : NB - 1);  // blocks of this role
   const int NN = g.m, K = g.k;
-  const int nb = (NN + XS - 1) / XS;
-  const int nblk = nb * (nb + 1) / 2;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int first = (wave + (int)(w & 3)) & 3;  // blocks first, first + 4
-  int bI[2], bJ[2];
-  bool has[2];
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int t = kq >> 1, comp = kq & 1;
+  f32x4 accR[3][2], accI[3][2];
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int b = first + 4 * q;
-    has[q] = b < nblk;
-    int P = 0;
-    while ((P + 1) * (P + 2) / 2 <= b) P++;
-    const int Q = b - P * (P + 1) / 2;  // Q <= P
-    bI[q] = (OUT == XO_TRI) ? P : Q;    // TRI: I rows <-> a; FULL: J rows <-> a (a >= b)
-    bJ[q] = (OUT == XO_TRI) ? Q : P;
-  }
-  // two accumulator sets per block (alternate sample pairs), summed at the
-  // end: four to eight independent MFMA chains per wave instead of two to four
-  f32x4 accR[2][2], accI[2][2];
-#pragma unroll
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 3; q++)
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
@@ -360,15 +354,34 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
         accR[q][h][r] = 0.f;
         accI[q][h][r] = 0.f;
       }
-  // rows NN..47 of every sample are zero for the whole launch
-  for (int i = threadIdx.x; i < XS_KC * XS_PLANE; i += 256) lds[i] = 0.f;
-  const int l16 = lane & 15, kq = lane >> 4;
-  const int t = kq >> 1, comp = kq & 1;
+  // x[h][gr]: sample s + 2h + t of row 16 gr + l16 (complex)
+  auto rd = [&](int s, float2 (&x)[2][NB]) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int gr = 0; gr < NB; gr++) x[h][gr] = lds[(s + 2 * h + t) * RL + XS * gr + l16];
+  };
+  auto mm = [&](const float2 (&x)[2][NB]) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float vv[NB], vt[NB];  // this lane's component, and (-im, re)
+#pragma unroll
+      for (int gr = 0; gr < NB; gr++) {
+        vv[gr] = comp ? x[h][gr].y : x[h][gr].x;
+        vt[gr] = comp ? x[h][gr].x : -x[h][gr].y;
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const int I = xs_blk_I<OUT, ROLE>(q), J = xs_blk_J<OUT, ROLE>(q);
+        accR[q][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(vv[I], vv[J], accR[q][h], 0, 0, 0);
+        accI[q][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(vt[I], vv[J], accI[q][h], 0, 0, 0);
+      }
+    }
+  };
 
-  const bool al16 = ((uintptr_t)base & 15) == 0;
   float4 v[XS_V4];
   if constexpr (LAYOUT == XL_KM)
-    xs_fetch(v, base, NN, 0, K, al16);
+    xs_fetch<AL16>(v, base, NN, 0, K);
   else
     xs_fetch_mk(v, base, g.ldI, g.xlgB, g.xbp, NN, 0, K);
   for (int s0 = 0; s0 < K; s0 += XS_KC) {
@@ -380,39 +393,30 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
     __syncthreads();
     if (s0 + XS_KC < K) {
       if constexpr (LAYOUT == XL_KM)
-        xs_fetch(v, base, NN, s0 + XS_KC, K, al16);
+        xs_fetch<AL16>(v, base, NN, s0 + XS_KC, K);
       else
         xs_fetch_mk(v, base, g.ldI, g.xlgB, g.xbp, NN, s0 + XS_KC, K);
     }
-#pragma unroll 2
-    for (int s = 0; s < XS_KC; s += 4) {
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const float *p = lds + (s + 2 * h + t) * XS_PLANE;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-          if (!has[q]) continue;
-          const int ri = bI[q] * XS + l16, rj = bJ[q] * XS + l16;
-          const float vi = p[comp * XS_RL + ri];
-          float vit = p[(1 - comp) * XS_RL + ri];
-          vit = comp ? vit : -vit;  // (-im, re)
-          const float vj = p[comp * XS_RL + rj];
-          accR[q][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(vi, vj, accR[q][h], 0, 0, 0);
-          accI[q][h] = __builtin_amdgcn_mfma_f32_16x16x4f32(vit, vj, accI[q][h], 0, 0, 0);
-        }
-      }
+    float2 x0[2][NB], x1[2][NB];
+    rd(0, x0);
+#pragma unroll 1
+    for (int s = 0; s < XS_KC; s += 8) {
+      rd(s + 4, x1);
+      mm(x0);
+      if (s + 8 < XS_KC) rd(s + 8, x0);
+      mm(x1);
     }
   }
 
   float2 *C = g.Ctab ? g.Ctab[batch] : g.Cbase + (long)batch * g.Cbatch;
   const float2 al = g.alpha, be = g.beta;
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    if (!has[q]) continue;
-    const int jg = bJ[q] * XS + l16;
+  for (int q = 0; q < NQ; q++) {
+    const int I = xs_blk_I<OUT, ROLE>(q), J = xs_blk_J<OUT, ROLE>(q);
+    const int jg = J * XS + l16;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int ig = bI[q] * XS + 4 * kq + r;
+      const int ig = I * XS + 4 * kq + r;
       const float dre = accR[q][0][r] + accR[q][1][r], dim = accI[q][0][r] + accI[q][1][r];
       if constexpr (OUT == XO_TRI) {
         const int a = ig, b = jg;  // V[a][b] = conj(D)
@@ -446,19 +450,57 @@ __global__ __launch_bounds__(256) void xeng_f32_small_kernel(XArgs g) {
   }
 }
 
+template <int LAYOUT, int OUT, int NB>
+__global__ __launch_bounds__(XS_NT, 4) void xeng_f32_small_kernel(XArgs g) {
+  constexpr int RL = XsRow<LAYOUT>::RL;
+  __shared__ float2 lds[XS_KC * RL];
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const long logical = (nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w;
+  const int batch = (int)logical;
+  const float2 *base;
+  if constexpr (OUT == XO_TRI)
+    base = g.Abase + xin_batch(g, batch);
+  else
+    base = g.Btab ? g.Btab[batch] : g.Bbase + (long)batch * g.Bbatch;
+  // rows NN..47 of every sample are zero for the whole launch
+  for (int i = threadIdx.x; i < XS_KC * RL; i += XS_NT) lds[i] = make_float2(0.f, 0.f);
+  const bool role0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0;
+  if (LAYOUT == XL_MK || ((uintptr_t)base & 15) == 0) {
+    if (role0)
+      xs_role<LAYOUT, OUT, 0, NB, true>(g, batch, base, lds);
+    else
+      xs_role<LAYOUT, OUT, 1, NB, true>(g, batch, base, lds);
+  } else {
+    if (role0)
+      xs_role<LAYOUT, OUT, 0, NB, false>(g, batch, base, lds);
+    else
+      xs_role<LAYOUT, OUT, 1, NB, false>(g, batch, base, lds);
+  }
+}
+
+template <int LAYOUT, int OUT>
+static int xs_launch(const XArgs &g, hipStream_t st) {
+  MDT_TRY(check_grid(g.batch, XS_NT, "X-engine"));
+  const int nb = (g.m + XS - 1) / XS;
+  if (nb == 1)
+    hipLaunchKernelGGL((xeng_f32_small_kernel<LAYOUT, OUT, 1>), dim3((unsigned)g.batch), dim3(XS_NT), 0, st, g);
+  else if (nb == 2)
+    hipLaunchKernelGGL((xeng_f32_small_kernel<LAYOUT, OUT, 2>), dim3((unsigned)g.batch), dim3(XS_NT), 0, st, g);
+  else
+    hipLaunchKernelGGL((xeng_f32_small_kernel<LAYOUT, OUT, 3>), dim3((unsigned)g.batch), dim3(XS_NT), 0, st, g);
+  return check_launch("xeng_f32_small_kernel");
+}
+
 int xeng_launch(const XArgs &g0, int layout, int out, hipStream_t st) {
   XArgs g = g0;
   if (layout == XL_KM && out == XO_FULL && g.herm && g.m <= XS_MAXN && (g.m & 1) == 0 && g.ldI == g.m) {
-    MDT_TRY(check_grid(g.batch, 256, "X-engine"));
-    hipLaunchKernelGGL((xeng_f32_small_kernel<XL_KM, XO_FULL>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
-    return check_launch("xeng_f32_small_kernel");
+    return xs_launch<XL_KM, XO_FULL>(g, st);
   }
   // the native triangle, every X-input layout (even pitches: 16-byte aligned pieces)
   if (layout == XL_MK && out == XO_TRI && g.herm && g.m <= XS_MAXN && (g.ldI & 1) == 0 && (g.xbp & 1) == 0 &&
       g.xlgB >= 1) {
-    MDT_TRY(check_grid(g.batch, 256, "X-engine"));
-    hipLaunchKernelGGL((xeng_f32_small_kernel<XL_MK, XO_TRI>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
-    return check_launch("xeng_f32_small_kernel");
+    return xs_launch<XL_MK, XO_TRI>(g, st);
   }
   const int nbM = (g.m + XT - 1) / XT;
   const int nbN = ((out == XO_TRI ? g.m : g.n) + XT - 1) / XT;
