@@ -597,8 +597,175 @@ __global__ __launch_bounds__(WC_THREADS, 2) void xeng_f16split_chan_kernel(XArgs
   }
 }
 
+// ---------------------------------------------------------------------------
+// Small arrays (NN <= 48, the MDT station's 40 inputs): 16 x 16 blocks on
+// v_mfma_f32_16x16x32_f16 (40 inputs pad to 48; the 128-input tile kernel's
+// 32 x 32 blocks cover 64 x 64 for three of its four waves).  At this size the
+// products take a fraction of the time of reading the X input once, so the
+// kernel is built to stream: one 128-thread workgroup per channel, 32-sample
+// chunks of all NN rows loaded as 16-byte pieces (16 lanes read a row's
+// 256 contiguous bytes) into registers one chunk ahead, then routed into hi
+// and lo planes [row][k] in LDS (k = 2 s + re/im; a 144-byte row pitch puts
+// the 16 rows of an operand read on distinct banks).  Block roles as in the
+// FP32 small-array kernel (xengine.hip), transposed for the triangle: with I
+// and J the A and B block rows and I >= J, role 0 = (0,0), (1,0), (2,0),
+// role 1 = (1,1), (2,1), (2,2), split once at the top.  Operand layout of the
+// 16 x 16 x 32 MFMA: lane l supplies A[l%16][8(l/16) .. +7] and
+// B[8(l/16) .. +7][l%16], i.e. samples 4(l/16) .. +3 of row l%16 as (re, im)
+// halves; D register r of lane l is D[4(l/16) + r][l%16].  The three split
+// products of a block go to one accumulator, interleaved over the role's
+// blocks so that dependent MFMAs are six apart.
+constexpr int XS16 = 16;                           // block edge
+constexpr int HS_KC = 32;                          // samples per chunk
+constexpr int HS_NT = 128;                         // threads
+constexpr int HS_MAXN = 48;
+constexpr int HS_ROWB = HS_KC * 4 + 16;            // bytes per LDS row per plane (128 data + 16 pad)
+constexpr int HS_PLANE = HS_MAXN * HS_ROWB;
+constexpr int HS_V4 = HS_MAXN * HS_KC / 2 / HS_NT;  // 16-byte pieces per thread per chunk (6)
+
+// piece q: samples s0 + 2(q%16), +1 of row q/16 as (hi0, lo0, hi1, lo1)
+// dwords; rows >= NN and samples >= K read as zero
+__device__ __forceinline__ void hs_fetch(float4 (&v)[HS_V4], const float2 *__restrict__ base, long ld, int lgB,
+                                         long bp, int NN, int s0, int K) {
+#pragma unroll
+  for (int i = 0; i < HS_V4; i++) {
+    const int q = threadIdx.x + i * HS_NT;
+    const int row = q >> 4, s = s0 + 2 * (q & 15);
+    if (row < NN && s < K) {
+      float4 x = *(const float4 *)(base + row * ld + xin_spec(lgB, bp, s));
+      if (s + 1 >= K) x.z = x.w = 0.f;
+      v[i] = x;
+    } else {
+      v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+__device__ __forceinline__ void hs_stage(const float4 (&v)[HS_V4], char *lds, int NN) {
+#pragma unroll
+  for (int i = 0; i < HS_V4; i++) {
+    const int q = threadIdx.x + i * HS_NT;
+    const int row = q >> 4, s = 2 * (q & 15);
+    if (row < NN) {
+      *(uint2 *)(lds + row * HS_ROWB + s * 4) = make_uint2(__float_as_uint(v[i].x), __float_as_uint(v[i].z));
+      *(uint2 *)(lds + HS_PLANE + row * HS_ROWB + s * 4) =
+          make_uint2(__float_as_uint(v[i].y), __float_as_uint(v[i].w));
+    }
+  }
+}
+
+template <int ROLE>
+__device__ __forceinline__ constexpr int hs_blk_I(int q) {
+  return ROLE == 0 ? q : (q == 0 ? 1 : 2);
+}
+template <int ROLE>
+__device__ __forceinline__ constexpr int hs_blk_J(int q) {
+  return ROLE == 0 ? 0 : (q == 2 ? 2 : 1);
+}
+
+template <int ROLE, int NB>
+__device__ __forceinline__ void hs_role(const XArgs &g, int batch, const float2 *__restrict__ base, char *lds,
+                                        float oscale) {
+  constexpr int NQ = ROLE == 0 ? NB : (NB == 3 ? 3 : NB - 1);
+  const int NN = g.m, K = g.k;
+  const int lane = threadIdx.x & 63, l16 = lane & 15, kq = lane >> 4;
+  f32x4 accR[3], accI[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      accR[q][r] = 0.f;
+      accI[q][r] = 0.f;
+    }
+  float4 v[HS_V4];
+  hs_fetch(v, base, g.ldI, g.xlgB, g.xbp, NN, 0, K);
+  for (int s0 = 0; s0 < K; s0 += HS_KC) {
+    __syncthreads();
+    hs_stage(v, lds, NN);
+    __syncthreads();
+    if (s0 + HS_KC < K) hs_fetch(v, base, g.ldI, g.xlgB, g.xbp, NN, s0 + HS_KC, K);
+#pragma unroll
+    for (int ks = 0; ks < HS_KC / 16; ks++) {
+      const int off = ks * 64 + kq * 16;
+      half8 H[NB], L[NB], Hr[NB], Lr[NB];
+#pragma unroll
+      for (int gr = 0; gr < NB; gr++) {
+        H[gr] = *(const half8 *)(lds + (XS16 * gr + l16) * HS_ROWB + off);
+        L[gr] = *(const half8 *)(lds + HS_PLANE + (XS16 * gr + l16) * HS_ROWB + off);
+        Hr[gr] = rot_conj(H[gr]);
+        Lr[gr] = rot_conj(L[gr]);
+      }
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          const int I = hs_blk_I<ROLE>(q), J = hs_blk_J<ROLE>(q);
+          const half8 a = p == 2 ? L[I] : H[I], ar = p == 2 ? Lr[I] : Hr[I], b = p == 1 ? L[J] : H[J];
+          accR[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, accR[q], 0, 0, 0);
+          accI[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, b, accI[q], 0, 0, 0);
+        }
+    }
+  }
+  // V[a][b] = conj(D[a][b]) * oscale, a = 16 I + 4 kq + r, b = 16 J + l16
+  const long cb = (long)batch * g.Cbatch;
+  double2 *acc = (double2 *)g.acc64;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int I = hs_blk_I<ROLE>(q), J = hs_blk_J<ROLE>(q);
+    const int b = XS16 * J + l16;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int a = XS16 * I + 4 * kq + r;
+      if (a < NN && b < NN && a >= b) {
+        const long o = cb + (long)a * (a + 1) / 2 + b;
+        const float vr = oscale * accR[q][r], vim = -oscale * accI[q][r];
+        g.Cbase[o] = make_float2(vr, vim);
+        if (acc) {
+          double2 tt = acc[o];
+          tt.x += (double)vr;
+          tt.y += (double)vim;
+          acc[o] = tt;
+        }
+      }
+    }
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(HS_NT) void xeng_f16split_small_kernel(XArgs g, float oscale) {
+  if (g.scale_dev != nullptr) {  // the scale the F-engine used (gains change it)
+    const float sc = *g.scale_dev;
+    oscale = g.alpha.x / (sc * sc);
+  }
+  __shared__ __attribute__((aligned(16))) char lds[2 * HS_PLANE];  // [hi|lo][row][k]
+  const long nwg = gridDim.x;
+  const long w = blockIdx.x;
+  const int batch = (int)((nwg % 8 == 0) ? (w % 8) * (nwg / 8) + w / 8 : w);
+  const float2 *base = g.Abase + xin_batch(g, batch);
+  // rows NN..47 stay zero for the whole launch
+  for (int i = threadIdx.x; i < 2 * HS_PLANE / 16; i += HS_NT) ((uint4 *)lds)[i] = make_uint4(0u, 0u, 0u, 0u);
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0)
+    hs_role<0, NB>(g, batch, base, lds, oscale);
+  else
+    hs_role<1, NB>(g, batch, base, lds, oscale);
+}
+
 int xeng_launch_f16split(const XArgs &g0, hipStream_t st) {
   XArgs g = g0;
+  if (g.m <= HS_MAXN && g.ldI == g.ldJ && g.Abase == g.Bbase && (g.ldI & 1) == 0 && (g.xbp & 1) == 0 &&
+      g.xlgB >= 1) {
+    MDT_TRY(check_grid(g.batch, HS_NT, "X-engine"));
+    const float sc = g.in_scale;
+    if (!(sc > 0.f)) return fail(MDT_ERR_INVALID_VALUE, "FP16 split needs a positive input scale");
+    const float oscale = g.alpha.x / (sc * sc);
+    const int nb = (g.m + XS16 - 1) / XS16;
+    if (nb == 1)
+      hipLaunchKernelGGL(xeng_f16split_small_kernel<1>, dim3((unsigned)g.batch), dim3(HS_NT), 0, st, g, oscale);
+    else if (nb == 2)
+      hipLaunchKernelGGL(xeng_f16split_small_kernel<2>, dim3((unsigned)g.batch), dim3(HS_NT), 0, st, g, oscale);
+    else
+      hipLaunchKernelGGL(xeng_f16split_small_kernel<3>, dim3((unsigned)g.batch), dim3(HS_NT), 0, st, g, oscale);
+    return check_launch("xeng_f16split_small_kernel");
+  }
   if (g.m > 128 && g.m <= WC_NB * 32 && g.ldI == g.ldJ && g.Abase == g.Bbase) {
     MDT_TRY(check_grid(g.batch, WC_THREADS, "X-engine"));
     const float sc = g.in_scale;
