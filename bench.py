@@ -17,7 +17,9 @@ sequence at the MDT station's shape with the FFT at lengths that are not
 powers of two (--no-station; --station-only runs that leg alone), the
 gain/delay calibration shape (--no-calib) and the HBM read / write / copy
 rates this device delivers to a streaming kernel, with the F-engine's time
-against the floor they set (--no-hbm-probe).
+against the floor they set (--no-hbm-probe).  --fengine-shapes runs only the
+F-engine across channel and tap counts (both of its kernels) and prints that
+table instead.
 """
 import argparse
 import json
@@ -589,6 +591,46 @@ def hbm_probe_leg(torch, mdtfx, nbytes=2684354560, grids=(1024, 2048, 4096, 8192
     return {"bytes_per_pass": nbytes, "unit": "GB/s", **best,
             "definition": "best over grids of 256-thread workgroups; read and copy count every byte moved"}
 
+# (inputs, channels, taps, spectra): about 1 GB of FP32 X input each
+FE_SHAPES = [(128, 64, 4, 16384), (64, 256, 8, 8192), (64, 512, 8, 4096), (64, 1024, 4, 2048),
+             (64, 1024, 16, 2048), (64, 2048, 6, 1024), (64, 4096, 8, 512), (64, 4096, 16, 512),
+             (64, 8192, 4, 256), (64, 16384, 4, 128), (64, 16384, 16, 128)]
+
+
+def fengine_shapes_leg(torch, mdtfx, reps=5):
+    """The F-engine alone (int8 in, FP32 or FP16 hi/lo X input out) across
+    channel and tap counts, each precision's default X-input layout, timed
+    with HIP events over `reps` launches after two untimed ones.  The library
+    runs the persistent kernel at 512-4096 channels (any tap count 4-128) and
+    the generic fused kernel elsewhere; "kernel" restates that rule."""
+    rows = []
+    for nn, n, t, ns in FE_SHAPES:
+        log(f"bench: F-engine shape NN={nn} N={n} T={t} Ns={ns}")
+        x8 = mdtfx.synth_i8(nn, n, ns + t - 1, seed=5, sigma=16.0)
+        alg = nn * (ns + t - 1) * n * 2 + t * n * 4 + nn * ns * n * 8
+        row = {"inputs": nn, "channels": n, "taps": t, "spectra": ns,
+               "kernel": "persistent" if 512 <= n <= 4096 else "generic", "algorithmic_bytes": alg}
+        for name, xp in (("fp32", mdtfx.XPREC_FP32), ("fp16_split", mdtfx.XPREC_FP16_SPLIT)):
+            plan = mdtfx.FxPlan(nn, n, t, ns, mdtfx.WINDOW_BH4, xp)
+            xin = plan.alloc_xinput()
+            for _ in range(2):
+                plan.fengine(x8, xin)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                plan.fengine(x8, xin)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            row[name] = {"ms": ms, "GB/s": alg / ms / 1e6, "frac_hbm": alg / ms / 1e6 / HBM_PEAK_GBS}
+            del plan, xin
+        rows.append(row)
+        del x8
+        torch.cuda.empty_cache()
+    return {"unit": "GB/s of algorithmic bytes (int8 in + window + 8-byte samples out)", "shapes": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -604,6 +646,8 @@ def main():
     ap.add_argument("--station-only", action="store_true",
                     help="only run the drop-in station leg and print its JSON (for per-kernel profiles)")
     ap.add_argument("--no-hbm-probe", action="store_true", help="skip the HBM read/write/copy ceiling leg")
+    ap.add_argument("--fengine-shapes", action="store_true",
+                    help="only time the F-engine across channel and tap counts and print that table")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -629,6 +673,9 @@ def main():
         return
     if args.station_only:
         print(json.dumps(station_leg(np, torch, mdtfx, args.steps, args.warmup, 0)), flush=True)
+        return
+    if args.fengine_shapes:
+        print(json.dumps(fengine_shapes_leg(torch, mdtfx)), flush=True)
         return
     stream = torch.cuda.current_stream().cuda_stream
     log(f"bench: scaled array NN={NN} N={N} T={T} Ns={NS} steps={args.steps} warmup={args.warmup}")
