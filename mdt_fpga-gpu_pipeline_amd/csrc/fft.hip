@@ -4,7 +4,9 @@
 //
 // One workgroup holds FPB transforms of N points (N/P threads each, P points
 // per thread, see fft_core.hpp); rows are read and written straight from/to
-// global memory by the first/last Stockham pass, 64-lane contiguous.
+// global memory by the first/last Stockham pass, 64-lane contiguous.  At
+// N = 256 .. 4096 the workgroups are persistent and load their next rows
+// while transforming the current ones (fft_stream_kernel).
 #include <new>
 
 #include "fft_any.hpp"
@@ -34,10 +36,105 @@ __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_kernel(const float2 *i
   }
 }
 
+// Streaming form: persistent workgroups, each taking the groups of FPB rows
+// g = blockIdx.x, + gridDim.x, ...; group g + gridDim.x is loaded into
+// registers while group g is transformed, so the HBM reads stay in flight
+// through the passes instead of each workgroup's reads, transforms and
+// writes following one another.  For that nothing inside the transform may
+// wait for those loads: the exchanges use bare barriers (which wait for LDS
+// operations only) and the twiddles come from the per-pass LDS table, filled
+// once per workgroup from the same W_N^k table.  The butterflies, twiddle
+// values and operation order are fft_kernel's, so the bits are too.
+template <int N, bool INV>
+__global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_stream_kernel(const float2 *in, float2 *out,
+                                                                        long batch,
+                                                                        const float2 *__restrict__ tw) {
+  using C = FftCfg<N>;
+  constexpr int P = C::P, TPB = C::TPB;
+  __shared__ float2 lds[C::FPB * lds_entries<N>()];
+  __shared__ float2 twl[pass_tw_entries<N, P>()];
+  fill_pass_tw<N, P>(twl, tw, threadIdx.x, C::THREADS);
+  __syncthreads();
+  const int t = threadIdx.x % TPB;
+  const int f = threadIdx.x / TPB;
+  const long ngroups = (batch + C::FPB - 1) / C::FPB;
+  // Every step issues the same P loads and P stores, so that the wait for the
+  // prefetched rows counts only the stores behind them: a row past the batch
+  // (only in the last group, which also holds row batch - 1) reads row
+  // batch - 1 and stores its identical transform there, and a prefetch past
+  // the last group re-reads the current one (discarded).  Two named buffers,
+  // unrolled by two, so that no register copy of the rows in flight (which
+  // would wait for them) lands inside a transform.
+  auto load = [&](float2 (&xv)[P], long grp) {
+    grp = grp < ngroups ? grp : grp - gridDim.x;
+    long row = grp * C::FPB + f;
+    row = row < batch ? row : batch - 1;
+    const float2 *src = in + row * N + t;
+#pragma unroll
+    for (int m = 0; m < P; m++) xv[m] = src[m * TPB];
+  };
+  auto step = [&](float2 (&xv)[P], long grp) {
+    fft_block_tw<N, P, INV, true>(xv, t, lds + f * lds_entries<N>(), PassTw<P>{twl});
+    long row = grp * C::FPB + f;
+    row = row < batch ? row : batch - 1;
+    float2 *dst = out + row * N + t;
+#pragma unroll
+    for (int m = 0; m < P; m++) store_stream(dst + m * TPB, xv[m]);
+  };
+  const long G = gridDim.x;
+  float2 xa[P], xb[P];
+  long g = blockIdx.x;
+  if (g < ngroups) load(xa, g);
+  for (; g < ngroups; g += 2 * G) {  // uniform: every thread runs the same barriers
+    load(xb, g + G);
+    step(xa, g);
+    if (g + G >= ngroups) break;
+    load(xa, g + 2 * G);
+    step(xb, g + G);
+  }
+}
+
+// resident workgroups of fft_stream_kernel<N, INV> on the current device
+// (CUs x occupancy), queried once per device; 0 if the query fails
+template <int N, bool INV>
+static long stream_grid() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cached[dev] == 0) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fft_stream_kernel<N, INV>, FftCfg<N>::THREADS, 0) !=
+            hipSuccess ||
+        cus <= 0 || per <= 0)
+      return 0;
+    cached[dev] = cus * per;
+  }
+  return cached[dev];
+}
+
+template <int N, bool INV>
+static int launch_fft_stream(const float2 *in, float2 *out, long batch, const float2 *tw, long grid,
+                             hipStream_t st) {
+  using C = FftCfg<N>;
+  hipLaunchKernelGGL((fft_stream_kernel<N, INV>), dim3((unsigned)grid), dim3(C::THREADS), 0, st, in, out, batch,
+                     tw);
+  return check_launch("fft_stream_kernel");
+}
+
 template <int N>
 static int launch_fft(const float2 *in, float2 *out, long batch, const float2 *tw, bool inv,
                       hipStream_t st) {
   using C = FftCfg<N>;
+  if constexpr (N >= 256 && N <= 4096) {
+    const long groups = (batch + C::FPB - 1) / C::FPB;
+    const long grid = inv ? stream_grid<N, true>() : stream_grid<N, false>();
+    if (grid > 0) {
+      const long g = grid < groups ? grid : groups;
+      return inv ? launch_fft_stream<N, true>(in, out, batch, tw, g, st)
+                 : launch_fft_stream<N, false>(in, out, batch, tw, g, st);
+    }
+  }
   long blocks = (batch + C::FPB - 1) / C::FPB;
   MDT_TRY(check_grid(blocks, C::THREADS, "FFT batch"));
   if (inv)

@@ -40,13 +40,21 @@ constexpr int fft_points(int N) { return (N == 1024 || N == 2048) ? 16 : (N <= 4
 // part of one) only orders its own LDS accesses (a wave's LDS operations
 // execute in order; the wavefront-scope fence keeps the compiler from moving
 // one lane's accesses past another's); larger transforms span waves and
-// need the workgroup barrier
-template <int TPB>
+// need the workgroup barrier.  RAW: a bare s_barrier after this wave's LDS
+// operations have completed (lgkmcnt(0)), which, unlike __syncthreads() and
+// its release fence, does not wait for the global loads in flight (used where
+// the next rows are being loaded during the transform)
+template <int TPB, bool RAW = false>
 __device__ __forceinline__ void fft_exchange_sync() {
   if constexpr (TPB <= 64) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else if constexpr (RAW) {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unchanged
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
   } else {
     __syncthreads();
   }
@@ -160,6 +168,29 @@ constexpr int pass_tw_entries() {  // radix-P passes hold P slots of NS, the rem
   constexpr int RL = N / ipow_c(P, L);
   return RL > 1 ? pass_tw_offset<P>(L) + ipow_c(P, L) * RL : pass_tw_offset<P>(L);
 }
+// fills the per-pass table below from the W_N^k table: pass p >= 1 (stage
+// NS = P^p, radix R) holds W_{NS*R}^{jm*r} = W_N^{jm*r*N/(NS*R)} at
+// [offset(p) + r*NS + jm]; entry idx < COUNT (the table's size: the whole
+// table, or the passes before the last where the last pass's twiddles are
+// held in registers) is written by thread idx mod nthreads
+template <int N, int P, int COUNT = pass_tw_entries<N, P>()>
+__device__ __forceinline__ void fill_pass_tw(float2 *tab, const float2 *tw, int tid, int nthreads) {
+  static_assert(COUNT >= 0 && COUNT <= pass_tw_entries<N, P>(), "table size");
+  constexpr int L = full_passes(N, P);
+  constexpr int RL = N / ipow_c(P, L);
+  constexpr int NPASS = L + (RL > 1 ? 1 : 0);
+  for (int idx = tid; idx < COUNT; idx += nthreads) {
+    int pass = 1, off = 0, ns = P;
+    while (pass < NPASS - 1 && idx >= off + ns * P) {
+      off += ns * P;
+      ns *= P;
+      pass++;
+    }
+    const int R = (pass < L) ? P : RL;
+    const int rel = idx - off, r = rel / ns, jm = rel % ns;
+    tab[idx] = tw[jm * r * (N / (ns * R))];
+  }
+}
 template <int P>
 struct PassTw {
   const float2 *tab;  // LDS
@@ -232,7 +263,7 @@ __device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds, con
   }
 }
 
-template <int N, int P, bool INV, int PASS, typename TW>
+template <int N, int P, bool INV, int PASS, bool RAW = false, typename TW>
 __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
   constexpr int L = full_passes(N, P);
   constexpr int RL = N / ipow_c(P, L);
@@ -241,7 +272,7 @@ __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, co
   constexpr int NS = ipow_c(P, PASS);
   constexpr int TPB = N / P;
   if constexpr (PASS > 0) {
-    fft_exchange_sync<TPB>();
+    fft_exchange_sync<TPB, RAW>();
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (TPB % 16 == 0) {  // lpad(t + m*TPB) = lpad(t) + m*TPB*17/16
       const float2 *rp = lds + lpad(t);
@@ -251,10 +282,10 @@ __device__ __forceinline__ void fft_stage(float2 (&x)[P], int t, float2 *lds, co
 #pragma unroll
       for (int m = 0; m < P; m++) x[m] = lds[lpad(t + m * TPB)];
     }
-    fft_exchange_sync<TPB>();
+    fft_exchange_sync<TPB, RAW>();
   }
   fft_pass<N, P, R, NS, PASS == NPASS - 1, INV>(x, t, lds, tw);
-  if constexpr (PASS + 1 < NPASS) fft_stage<N, P, INV, PASS + 1>(x, t, lds, tw);
+  if constexpr (PASS + 1 < NPASS) fft_stage<N, P, INV, PASS + 1, RAW>(x, t, lds, tw);
 }
 
 // NX independent transforms advanced together: each pass does the NX sets of
@@ -343,10 +374,10 @@ struct LastRegTw {
 // threads of the block must call this the same number of times (it contains
 // __syncthreads when the transform spans more than one wave).  lds:
 // lds_entries<N>() float2 private to this transform.
-template <int N, int P, bool INV, typename TW>
+template <int N, int P, bool INV, bool RAW = false, typename TW>
 __device__ __forceinline__ void fft_block_tw(float2 (&x)[P], int t, float2 *lds, const TW &tw) {
   static_assert(N % P == 0 && N >= P, "N must be a multiple of P");
-  fft_stage<N, P, INV, 0>(x, t, lds, tw);
+  fft_stage<N, P, INV, 0, RAW>(x, t, lds, tw);
 }
 template <int N, int P, bool INV>
 __device__ __forceinline__ void fft_block(float2 (&x)[P], int t, float2 *lds,
