@@ -353,10 +353,6 @@ __device__ __forceinline__ void stage_barrier() {
 //  lanes: each lane's chunk offsets are computed once per item instead of
 //         in every store round (40 % fewer integer instructions; FP16 hi/lo
 //         with pin, FP8 12.9-13.1 against 13.1 ms; FP32 10.2 against 9.8 ms);
-//  byte1: the input's high byte taken from the top of the word by an
-//         arithmetic shift, which converts with one SDWA byte-1 conversion
-//         instead of a 16-bit shift and a conversion (FP16 hi/lo 8.52
-//         against 8.62 ms; FP32 10.0 against 9.8-9.9 ms, FP8 unchanged);
 //  stream: the X input written with non-temporal (streaming) 16-byte stores:
 //         it is read back only after the whole launch, far more data than
 //         the caches hold (FP16 hi/lo 8.35 against 8.46 ms, FP8 12.14
@@ -367,9 +363,19 @@ template <int SPLIT>
 struct FeTuning {  // split 3 (e4m3) follows split 2's choices
   static constexpr bool pin = SPLIT == 1;
   static constexpr bool lanes = SPLIT != 0;
-  static constexpr bool byte1 = SPLIT == 1;
   static constexpr bool stream = SPLIT != 0;
 };
+
+// an int8 (I, Q) pair in the low 16 bits of u -> two floats, one SDWA
+// conversion of a sign-extended byte each (exact).  Written out because the
+// compiler reaches one byte through a bit-field extract or a shift first:
+// four VALU instructions per pair instead of two.
+__device__ __forceinline__ float2 i8pair_f32(unsigned u) {
+  float re, im;
+  asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0" : "=v"(re) : "v"(u));
+  asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1" : "=v"(im) : "v"(u));
+  return make_float2(re, im);
+}
 
 // one 16-byte store of four 32-bit words, streaming or not
 template <bool STREAM>
@@ -558,7 +564,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
             const int fr = ff < fmax ? ff : fmax;
             const unsigned short u =
                 __builtin_amdgcn_raw_buffer_load_b16(rin, 2 * (tl + m * TPB), (int)(2 * (long)fr * spec), 0);
-            x[f] = make_float2((float)(signed char)(u & 0xff), (float)(signed char)(u >> 8));
+            x[f] = i8pair_f32(u);
           }
 #pragma unroll
           for (int sl = 0; sl < SB; sl++)
@@ -596,11 +602,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
           u = ((const unsigned short *)frb)[(SB * q + f) * N + tl + m * TPB];
         else
           u = __builtin_amdgcn_raw_buffer_load_b16(rin, 2 * (tl + m * TPB), (int)(2 * (long)fr * spec), 0);
-        // (I, Q) = sign-extended low and high bytes
-        if constexpr (FeTuning<SPLIT>::byte1)
-          x[f] = make_float2((float)(signed char)u, (float)((int)((unsigned)u << 16) >> 24));
-        else
-          x[f] = make_float2((float)(signed char)(u & 0xff), (float)(signed char)(u >> 8));
+        x[f] = i8pair_f32(u);  // (I, Q) = sign-extended low and high bytes
       }
 #pragma unroll
       for (int sl = 0; sl < SB; sl++) {
