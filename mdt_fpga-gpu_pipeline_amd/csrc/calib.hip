@@ -88,7 +88,7 @@ __global__ __launch_bounds__(CAL_THREADS) void calib_apply_kernel(int NN, int N,
 // sums, as its two terms.  Lane z then adds 2 S conj(phi_z(c)) and 2 f_c T
 // to its channel sums.  Fixed order throughout: compensated per-lane sums, a
 // butterfly over each row, the four waves combined in order; partials per
-// (input, chunk) and per block, summed in order by calib_finish_kernel.
+// (input, chunk) and per block, summed in a fixed order by calib_finish_kernel.
 template <bool GRAD>
 __global__ __launch_bounds__(CAL_THREADS) void calib_lg_kernel(int NN, int N, long nb, int nchunk,
                                                                const float2 *g, const float *tau,
@@ -182,27 +182,82 @@ __global__ __launch_bounds__(CAL_THREADS) void calib_lg_kernel(int NN, int N, lo
   }
 }
 
-// final fixed-order sums: thread z over its chunk partials; thread 0 the loss
-__global__ void calib_finish_kernel(int NN, int nchunk, int nlossblk, const float *gpart,
-                                    const float *lpart, float *loss, float2 *grad_g,
-                                    float *grad_tau) {
-  for (int z = threadIdx.x; z < NN; z += blockDim.x) {
-    if (grad_g) {
-      Kahan a, b, t;
-      for (int k = 0; k < nchunk; k++) {
-        const float *o = gpart + ((long)z * nchunk + k) * 3;
-        a.add(o[0]);
-        b.add(o[1]);
-        t.add(o[2]);
+// Final sums in a fixed order (the same bits on every run).  Gradients: 16
+// threads per input, thread `sub` summing chunks sub, sub + 16, ... (all its
+// loads issued before the first add), then the input's first thread adds the
+// 16 partial sums in order.  Loss: every thread sums a strided share of the
+// block partials, and the 1024 sums are combined in two ordered levels.
+constexpr int CAL_FT = 1024, CAL_FSUB = 16, CAL_FLD = 16;  // threads, threads per input, loads per thread
+__global__ __launch_bounds__(CAL_FT) void calib_finish_kernel(int NN, int nchunk, int nlossblk, const float *gpart,
+                                                             const float *lpart, float *loss, float2 *grad_g,
+                                                             float *grad_tau) {
+  __shared__ float sh[3][CAL_FT];
+  const int tid = threadIdx.x, sub = tid % CAL_FSUB;
+  if (grad_g) {
+    for (int z0 = 0; z0 < NN; z0 += CAL_FT / CAL_FSUB) {  // uniform
+      const int z = z0 + tid / CAL_FSUB;
+      const int zc = z < NN ? z : NN - 1;
+      float va[CAL_FLD], vb[CAL_FLD], vt[CAL_FLD];
+#pragma unroll
+      for (int i = 0; i < CAL_FLD; i++) {
+        const int k = sub + CAL_FSUB * i;
+        const float *o = gpart + ((long)zc * nchunk + (k < nchunk ? k : nchunk - 1)) * 3;  // in range
+        va[i] = o[0];
+        vb[i] = o[1];
+        vt[i] = o[2];
       }
-      grad_g[z] = make_float2(a.value(), b.value());
-      if (grad_tau) grad_tau[z] = t.value();
+      Kahan a, b, t;
+#pragma unroll
+      for (int i = 0; i < CAL_FLD; i++) {
+        if (sub + CAL_FSUB * i < nchunk) {
+          a.add(va[i]);
+          b.add(vb[i]);
+          t.add(vt[i]);
+        }
+      }
+      sh[0][tid] = a.value();
+      sh[1][tid] = b.value();
+      sh[2][tid] = t.value();
+      __syncthreads();
+      if (sub == 0 && z < NN) {
+        Kahan fa, fb, ft;
+        for (int j = 0; j < CAL_FSUB; j++) {
+          fa.add(sh[0][tid + j]);
+          fb.add(sh[1][tid + j]);
+          ft.add(sh[2][tid + j]);
+        }
+        grad_g[z] = make_float2(fa.value(), fb.value());
+        if (grad_tau) grad_tau[z] = ft.value();
+      }
+      __syncthreads();
     }
   }
-  if (threadIdx.x == 0 && loss) {
+  if (loss) {
     Kahan l;
-    for (int k = 0; k < nlossblk; k++) l.add(lpart[k]);
-    *loss = l.value();
+    for (int k0 = 0; k0 < nlossblk; k0 += CAL_FT * CAL_FLD) {  // uniform
+      float v[CAL_FLD];
+#pragma unroll
+      for (int i = 0; i < CAL_FLD; i++) {
+        const int k = k0 + tid + CAL_FT * i;
+        v[i] = lpart[k < nlossblk ? k : nlossblk - 1];
+      }
+#pragma unroll
+      for (int i = 0; i < CAL_FLD; i++)
+        if (k0 + tid + CAL_FT * i < nlossblk) l.add(v[i]);
+    }
+    sh[0][tid] = l.value();
+    __syncthreads();
+    if (tid < 32) {
+      Kahan m;
+      for (int j = 0; j < 32; j++) m.add(sh[0][tid * 32 + j]);
+      sh[1][tid] = m.value();
+    }
+    __syncthreads();
+    if (tid == 0) {
+      Kahan m;
+      for (int j = 0; j < 32; j++) m.add(sh[1][j]);
+      *loss = m.value();
+    }
   }
 }
 
@@ -213,6 +268,7 @@ struct CalGeom {
   size_t lds;
 };
 constexpr int CAL_MAX_CHUNKS = 256;
+static_assert(CAL_MAX_CHUNKS <= CAL_FSUB * CAL_FLD, "the finish kernel's loads cover every chunk");
 __host__ inline CalGeom cal_geom(int NN, int N) {
   CalGeom q;
   q.nchunk = (N + 3) / 4 < CAL_MAX_CHUNKS ? (N + 3) / 4 : CAL_MAX_CHUNKS;
@@ -249,7 +305,7 @@ int calib_loss_grad_run(int NN, int N, const float2 *V, const float2 *M, const f
     hipLaunchKernelGGL(calib_lg_kernel<false>, grid, dim3(CAL_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V,
                        M, gpart, lpart);
   MDT_TRY(check_launch("calib_lg_kernel"));
-  hipLaunchKernelGGL(calib_finish_kernel, dim3(1), dim3(256), 0, st, NN, q.nchunk, q.nchunk * q.ntile, gpart, lpart,
+  hipLaunchKernelGGL(calib_finish_kernel, dim3(1), dim3(CAL_FT), 0, st, NN, q.nchunk, q.nchunk * q.ntile, gpart, lpart,
                      loss, grad_g, grad_tau);
   return check_launch("calib_finish_kernel");
 }

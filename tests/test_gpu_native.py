@@ -158,7 +158,8 @@ def test_xinput_block_geometry(layout, xprec, N, T):
     assert np.all(raw[tail] == filler)
 
 
-@pytest.mark.parametrize("NN,N,T,Ns", [(3, 32, 4, 2), (65, 256, 4, 70), (130, 128, 8, 40)])
+@pytest.mark.parametrize("NN,N,T,Ns", [(3, 32, 4, 2), (65, 256, 4, 70), (130, 128, 8, 40),
+                                       (57, 256, 4, 70), (64, 128, 4, 33), (33, 64, 4, 20)])
 def test_native_xengine_edges(NN, N, T, Ns):
     """tile edges (NN not a multiple of 64, Ns not a multiple of 32)"""
     print(f"native NN={NN} N={N} T={T} Ns={Ns}", flush=True)

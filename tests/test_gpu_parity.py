@@ -163,7 +163,8 @@ def _autonorm_err(V, V64):
 
 
 @pytest.mark.parametrize("NN,Ns,N", [(3, 2, 32), (38, 64, 32), (40, 256, 32), (64, 33, 32), (100, 129, 32), (130, 40, 32),
-                                     (40, 100, 32), (48, 33, 32), (4, 70, 32)])  # partial 64-sample chunks, NN 48 / 4
+                                     (40, 100, 32), (48, 33, 32), (4, 70, 32),  # partial chunks, NN 48 / 4
+                                     (50, 70, 32), (58, 64, 32), (18, 40, 32)])  # three waves (NN 49-64), two blocks
 def test_herk_vs_fp64(NN, Ns, N):
     print(f"herk NN={NN} Ns={Ns} N={N}", flush=True)
     rng = _rng(NN + Ns)
@@ -176,7 +177,7 @@ def test_herk_vs_fp64(NN, Ns, N):
     assert _autonorm_err(V, V64) < 1e-5
 
 
-@pytest.mark.parametrize("NN,Ns,N", [(38, 1024, 4), (40, 64, 6), (5, 3, 2)])
+@pytest.mark.parametrize("NN,Ns,N", [(38, 1024, 4), (40, 64, 6), (5, 3, 2), (64, 96, 3), (52, 40, 2)])
 def test_cgemm_batched_reference_call(NN, Ns, N):
     """The exact reference call: cublasCgemmBatched(h, OP_N, OP_C, NN, NN, Ns,
     1/Ns, XInPtr, NN, XInPtr, NN, 0, XOutPtr, NN, N) (client.cu:656-660, 1128)."""
