@@ -119,6 +119,33 @@ def test_fft_vs_fp64(N):
     plan.destroy()
 
 
+@pytest.mark.parametrize("N,fpb", [(256, 8), (512, 4), (1024, 4), (2048, 2), (4096, 1)])
+def test_fft_streaming_many_rows(N, fpb):
+    """Batches large enough that each persistent workgroup runs several groups
+    of rows, with a partial last group (batch = 3 * 1024 groups * fpb + 1):
+    sampled rows against the FP64 oracle, forward out of place and inverse in
+    place, and the same input row transformed bit-identically in the first
+    group and in the partial last one."""
+    batch = 3 * 1024 * fpb + 1
+    print(f"fft stream N={N} batch={batch}", flush=True)
+    rng = _rng(N + 7)
+    x = (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N))).astype(np.complex64)
+    x[-1] = x[0]
+    plan = mdtfx.FftPlan(N, batch)
+    plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    xi = _to_gpu(x)
+    X = _host(plan.exec_c2c(xi, torch.empty_like(xi), mdtfx.FFT_FORWARD))
+    rows = np.unique(np.concatenate([[0, 1, batch - 2, batch - 1], rng.integers(0, batch, 60)]))
+    ref = orc.fft_f64(x[rows].astype(np.complex128), -1)
+    assert _fft_err(X[rows], ref) < 2.5e-7 * np.log2(N)
+    assert np.array_equal(X[0].view(np.uint32), X[-1].view(np.uint32))
+    Xi = _host(plan.exec_c2c(xi, xi, mdtfx.FFT_INVERSE))
+    refi = orc.fft_f64(x[rows].astype(np.complex128), +1)
+    assert _fft_err(Xi[rows], refi) < 2.5e-7 * np.log2(N)
+    assert np.array_equal(Xi[0].view(np.uint32), Xi[-1].view(np.uint32))
+    plan.destroy()
+
+
 def test_fft_numpy_golden(golden):
     for N in [8, 32, 1024, 2048, 4096]:
         x = golden[f"fft_in_N{N}"].astype(np.complex64)
