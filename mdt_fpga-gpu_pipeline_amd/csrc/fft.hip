@@ -7,6 +7,7 @@
 // global memory by the first/last Stockham pass, 64-lane contiguous.  At
 // N = 256 .. 4096 the workgroups are persistent and load their next rows
 // while transforming the current ones (fft_stream_kernel).
+#include <atomic>
 #include <new>
 
 #include "fft_any.hpp"
@@ -95,22 +96,25 @@ __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_stream_kernel(const fl
 }
 
 // resident workgroups of fft_stream_kernel<N, INV> on the current device
-// (CUs x occupancy), queried once per device; 0 if the query fails
+// (CUs x occupancy), queried once per device (threads that race here store
+// the same value); 0 if the query fails
 template <int N, bool INV>
 static long stream_grid() {
-  static int cached[64] = {};
+  static std::atomic<int> cached[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
-  if (cached[dev] == 0) {
+  int grid = cached[dev].load(std::memory_order_relaxed);
+  if (grid == 0) {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fft_stream_kernel<N, INV>, FftCfg<N>::THREADS, 0) !=
             hipSuccess ||
         cus <= 0 || per <= 0)
       return 0;
-    cached[dev] = cus * per;
+    grid = cus * per;
+    cached[dev].store(grid, std::memory_order_relaxed);
   }
-  return cached[dev];
+  return grid;
 }
 
 template <int N, bool INV>
