@@ -482,20 +482,32 @@ def load_traffic():
 
 
 def _cpu_model():
+    """lscpu's model name, with family / model / stepping and AVX-512
+    presence (virtualised hosts may report only a generic name)"""
     import subprocess
+    fields = {}
     try:
         for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
-            if line.startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
     except (OSError, subprocess.SubprocessError):
         pass
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+    if "Model name" not in fields:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    k = {"model name": "Model name", "cpu family": "CPU family", "model": "Model",
+                         "stepping": "Stepping", "flags": "Flags"}.get(k.strip())
+                    if k and k not in fields:
+                        fields[k] = v.strip()
+        except OSError:
+            pass
+    name = fields.get("Model name", "unknown")
+    fms = "/".join(fields.get(k, "?") for k in ("CPU family", "Model", "Stepping"))
+    avx512 = "avx512f" in fields.get("Flags", "").split()
+    return f"{name} (family/model/stepping {fms}, AVX-512 {'yes' if avx512 else 'no'})"
 
 
 def cpu_baseline_and_error(np, torch, plan, x8, vis16, vis32, sample_spectra, err_channels):
