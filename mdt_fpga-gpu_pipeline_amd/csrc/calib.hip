@@ -35,7 +35,8 @@ struct Kahan {
 
 __device__ __forceinline__ long bl_of(int a, int b) { return (long)a * (a + 1) / 2 + b; }
 
-constexpr int CAL_THREADS = 256, CAL_WAVES = 4, CAL_TILE = 64;
+constexpr int CAL_THREADS = 256, CAL_TILE = 64;      // apply kernel threads, inputs per tile
+constexpr int CAL_LG_THREADS = 1024, CAL_WAVES = 16;  // loss/adjoint kernel
 
 // sum over the 64 lanes of a wave by an xor butterfly: every lane ends with
 // the same bits (each step adds the same two values in either order)
@@ -73,94 +74,121 @@ __global__ __launch_bounds__(CAL_THREADS) void calib_apply_kernel(int NN, int N,
   }
 }
 
-// Loss and adjoint partials.  Block (channel chunk, input tile of 64): wave w
-// takes channels c0 + w, c0 + w + 4, ... of the chunk; lane l is input
-// z = 64 tile + l.  Per channel, with the channel's gains G in the wave's LDS
-// row and R = V' - M:
-//   rows:    for each input zz of the tile, its row of baselines (zz, p <= zz)
-//            (contiguous, lanes over p): |R|^2 into the loss, and
-//            R G_p conj(V), Im(conj(R) V') summed over the row for input zz;
-//   columns: for each input p >= the tile's first, its row's baselines
-//            (p, z) with z in the tile (contiguous, lane z): conj(R) G_p V
-//            and -Im(conj(R) V') for input z.
+// Loss and adjoint partials.  Block (channel chunk, input tile of 64):
+// CAL_CH channels of the chunk in flight, CAL_CW waves on each (slot cs =
+// w / CAL_CW, part pw = w % CAL_CW); lane l is input z = 64 tile + l.  Per
+// channel, with the channel's gains G in the slot's LDS row and R = V' - M:
+//   rows:    for each input zz of the tile with (zz - z0) % CAL_CW == pw, its
+//            row of baselines (zz, p <= zz) (contiguous, lanes over p): |R|^2
+//            into the loss, and R G_p conj(V), Im(conj(R) V') summed over the
+//            row for input zz;
+//   columns: for each input p >= the tile's first with (p - z0) % CAL_CW ==
+//            pw, its row's baselines (p, z) with z in the tile (contiguous,
+//            lane z): conj(R) G_p V and -Im(conj(R) V') for input z.
 // Every baseline enters the loss once (rows partition the triangle) and the
 // gradient of each of its two inputs once; the diagonal (z, z) enters both
-// sums, as its two terms.  Lane z then adds 2 S conj(phi_z(c)) and 2 f_c T
-// to its channel sums.  Fixed order throughout: compensated per-lane sums, a
-// butterfly over each row, the four waves combined in order; partials per
-// (input, chunk) and per block, summed in a fixed order by calib_finish_kernel.
+// sums, as its two terms.  The slot's first wave adds its waves' sums for
+// lane z in order, then 2 S conj(phi_z(c)) and 2 f_c T to its channel sums.
+// Fixed order throughout: compensated per-lane sums, a butterfly over each
+// row, the waves combined in order; partials per (input, chunk) and per
+// block, summed in a fixed order by calib_finish_kernel.  Splitting each
+// channel over four waves keeps four waves per SIMD in flight at the
+// calibration shape (64 inputs, 1024 channels: one wave per channel left one
+// wave per SIMD, latency-bound: 0.092 ms).
+constexpr int CAL_CW = 4, CAL_CH = CAL_WAVES / CAL_CW;  // waves per channel, channels in flight
 template <bool GRAD>
-__global__ __launch_bounds__(CAL_THREADS) void calib_lg_kernel(int NN, int N, long nb, int nchunk,
+__global__ __launch_bounds__(CAL_LG_THREADS) void calib_lg_kernel(int NN, int N, long nb, int nchunk,
                                                                const float2 *g, const float *tau,
                                                                const float2 *V, const float2 *M,
                                                                float *gpart, float *lpart) {
-  extern __shared__ float2 Gs[];  // [CAL_WAVES][NN]
+  extern __shared__ float2 Gs[];  // [CAL_CH][NN]
+  __shared__ float part[CAL_WAVES][3][64];
   __shared__ float red[CAL_WAVES][4][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cs = w / CAL_CW, pw = w % CAL_CW;
   const int z0 = blockIdx.y * CAL_TILE, z = z0 + lane;
   const int zend = min(z0 + CAL_TILE, NN);
   const int cpc = (N + nchunk - 1) / nchunk;
   const int c0 = blockIdx.x * cpc, c1 = min(N, c0 + cpc);
-  float2 *Gw = Gs + (long)w * NN;
+  float2 *Gw = Gs + (long)cs * NN;
   Kahan sgr, sgi, stt, sl;
-  for (int c = c0 + w; c < c1; c += CAL_WAVES) {
-    wave_gains(Gw, NN, N, c, g, tau, lane);
-    const long cb = (long)c * nb;
-    Kahan ar, ai, at;  // lane z's row sums (set by the butterflies)
-    for (int zz = z0; zz < zend; zz++) {
-      const float2 Gzz = Gw[zz];
-      const long rb = cb + bl_of(zz, 0);
-      Kahan rr, ri, rt;
-      for (int p = lane; p <= zz; p += 64) {
-        const float2 v = V[rb + p], m = M[rb + p];
-        const float2 Gp = Gw[p];
-        const float2 vp = cmul(cmulc(Gzz, Gp), v);
-        const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
-        sl.add(__builtin_fmaf(r.x, r.x, r.y * r.y));
+  const int nround = (cpc + CAL_CH - 1) / CAL_CH;  // uniform over the block
+  for (int k = 0; k < nround; k++) {
+    const int c = c0 + cs + k * CAL_CH;
+    const bool cv = c < c1;  // uniform over the slot's waves
+    __syncthreads();  // the previous round's reads of the gains are done
+    if (cv) {
+      const int kk = signed_bin(c, N);
+      for (int a = pw * 64 + lane; a < NN; a += CAL_CW * 64) Gw[a] = cmul(g[a], delay_phasor(kk, tau ? tau[a] : 0.f, N));
+    }
+    __syncthreads();
+    Kahan ar, ai, at;  // this wave's part of lane z's sums
+    if (cv) {
+      const long cb = (long)c * nb;
+      for (int zz = z0 + pw; zz < zend; zz += CAL_CW) {
+        const float2 Gzz = Gw[zz];
+        const long rb = cb + bl_of(zz, 0);
+        Kahan rr, ri, rt;
+        for (int p = lane; p <= zz; p += 64) {
+          const float2 v = V[rb + p], m = M[rb + p];
+          const float2 Gp = Gw[p];
+          const float2 vp = cmul(cmulc(Gzz, Gp), v);
+          const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
+          sl.add(__builtin_fmaf(r.x, r.x, r.y * r.y));
+          if constexpr (GRAD) {
+            const float2 t = cmulc(cmul(r, Gp), v);
+            rr.add(t.x);
+            ri.add(t.y);
+            if (p != zz) rt.add(__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+          }
+        }
         if constexpr (GRAD) {
-          const float2 t = cmulc(cmul(r, Gp), v);
-          rr.add(t.x);
-          ri.add(t.y);
-          if (p != zz) rt.add(__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+          const float sr = wave_sum(rr.value()), si = wave_sum(ri.value()), st = wave_sum(rt.value());
+          if (lane == zz - z0) {
+            ar.add(sr);
+            ai.add(si);
+            at.add(st);
+          }
         }
       }
       if constexpr (GRAD) {
-        const float sr = wave_sum(rr.value()), si = wave_sum(ri.value()), st = wave_sum(rt.value());
-        if (lane == zz - z0) {
-          ar.add(sr);
-          ai.add(si);
-          at.add(st);
+        const float2 Gz = z < NN ? Gw[z] : make_float2(0.f, 0.f);
+        for (int p = z0 + pw; p < NN; p += CAL_CW) {
+          if (z <= p) {
+            const float2 Gp = Gw[p];
+            const long i = cb + bl_of(p, z);
+            const float2 v = V[i], m = M[i];
+            const float2 vp = cmul(cmulc(Gp, Gz), v);
+            const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
+            const float2 t = cmul(make_float2(r.x, -r.y), cmul(Gp, v));
+            ar.add(t.x);
+            ai.add(t.y);
+            if (p != z) at.add(-__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+          }
         }
       }
     }
     if constexpr (GRAD) {
-      const float2 Gz = z < NN ? Gw[z] : make_float2(0.f, 0.f);
-      for (int p = z0; p < NN; p++) {
-        if (z <= p) {
-          const float2 Gp = Gw[p];
-          const long i = cb + bl_of(p, z);
-          const float2 v = V[i], m = M[i];
-          const float2 vp = cmul(cmulc(Gp, Gz), v);
-          const float2 r = make_float2(vp.x - m.x, vp.y - m.y);
-          const float2 t = cmul(make_float2(r.x, -r.y), cmul(Gp, v));
-          ar.add(t.x);
-          ai.add(t.y);
-          if (p != z) at.add(-__builtin_fmaf(r.x, vp.y, -(r.y * vp.x)));
+      part[w][0][lane] = ar.value();
+      part[w][1][lane] = ai.value();
+      part[w][2][lane] = at.value();
+      __syncthreads();
+      if (cv && pw == 0 && z < NN) {
+        Kahan sa, sb, st;
+        for (int q = 0; q < CAL_CW; q++) {
+          sa.add(part[w + q][0][lane]);
+          sb.add(part[w + q][1][lane]);
+          st.add(part[w + q][2][lane]);
         }
-      }
-      if (z < NN) {
         const int kk = signed_bin(c, N);
         const float f = 6.283185307179586f * (float)kk / (float)N;
         const float2 ph = delay_phasor(kk, tau ? tau[z] : 0.f, N);
-        const float2 t = cmulc(make_float2(ar.value(), ai.value()), ph);  // S conj(phi)
+        const float2 t = cmulc(make_float2(sa.value(), sb.value()), ph);  // S conj(phi)
         sgr.add(2.0f * t.x);
         sgi.add(2.0f * t.y);
-        stt.add(2.0f * f * at.value());
+        stt.add(2.0f * f * st.value());
       }
     }
-    // the next channel rewrites this wave's gains after every lane read them
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
   }
   red[w][0][lane] = sgr.value();
   red[w][1][lane] = sgi.value();
@@ -169,8 +197,11 @@ __global__ __launch_bounds__(CAL_THREADS) void calib_lg_kernel(int NN, int N, lo
   __syncthreads();
   if (w == 0) {
     Kahan q[4];
-    for (int ww = 0; ww < CAL_WAVES; ww++)
-      for (int k = 0; k < 4; k++) q[k].add(red[ww][k][lane]);
+    for (int ww = 0; ww < CAL_WAVES; ww++) {
+      if (ww % CAL_CW == 0)  // the slots' channel sums
+        for (int k = 0; k < 3; k++) q[k].add(red[ww][k][lane]);
+      q[3].add(red[ww][3][lane]);  // every wave's loss terms
+    }
     if (GRAD && z < NN) {
       float *o = gpart + ((long)z * nchunk + blockIdx.x) * 3;
       o[0] = q[0].value();
@@ -274,7 +305,7 @@ __host__ inline CalGeom cal_geom(int NN, int N) {
   q.nchunk = (N + 3) / 4 < CAL_MAX_CHUNKS ? (N + 3) / 4 : CAL_MAX_CHUNKS;
   q.ntile = (NN + CAL_TILE - 1) / CAL_TILE;
   q.nb = (long)NN * (NN + 1) / 2;
-  q.lds = sizeof(float2) * (size_t)CAL_WAVES * NN;
+  q.lds = sizeof(float2) * (size_t)CAL_CH * NN;
   return q;
 }
 // workspace in floats: gradient partials [NN][nchunk][3], then loss partials [nchunk][ntile]
@@ -291,18 +322,18 @@ int calib_loss_grad_run(int NN, int N, const float2 *V, const float2 *M, const f
   static bool attr_set = false;  // LDS above 64 KiB for the largest arrays
   if (!attr_set) {
     if (hipFuncSetAttribute((const void *)calib_lg_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(sizeof(float2) * CAL_WAVES * MDT_MAX_INPUTS)) != hipSuccess ||
+                            (int)(sizeof(float2) * CAL_CH * MDT_MAX_INPUTS)) != hipSuccess ||
         hipFuncSetAttribute((const void *)calib_lg_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(sizeof(float2) * CAL_WAVES * MDT_MAX_INPUTS)) != hipSuccess)
+                            (int)(sizeof(float2) * CAL_CH * MDT_MAX_INPUTS)) != hipSuccess)
       return fail(MDT_ERR_DEVICE, "LDS size of the calibration kernels");
     attr_set = true;
   }
   const dim3 grid(q.nchunk, q.ntile);
   if (grad_g)
-    hipLaunchKernelGGL(calib_lg_kernel<true>, grid, dim3(CAL_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V, M,
+    hipLaunchKernelGGL(calib_lg_kernel<true>, grid, dim3(CAL_LG_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V, M,
                        gpart, lpart);
   else
-    hipLaunchKernelGGL(calib_lg_kernel<false>, grid, dim3(CAL_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V,
+    hipLaunchKernelGGL(calib_lg_kernel<false>, grid, dim3(CAL_LG_THREADS), q.lds, st, NN, N, q.nb, q.nchunk, g, tau, V,
                        M, gpart, lpart);
   MDT_TRY(check_launch("calib_lg_kernel"));
   hipLaunchKernelGGL(calib_finish_kernel, dim3(1), dim3(CAL_FT), 0, st, NN, q.nchunk, q.nchunk * q.ntile, gpart, lpart,
