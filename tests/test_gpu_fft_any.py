@@ -56,6 +56,38 @@ def test_fft_any_length_vs_fp64(N):
     plan.destroy()
 
 
+@pytest.mark.parametrize("N,fpb", [(243, 8), (1000, 2), (1536, 1), (3000, 1), (6561, 1), (7000, 1), (8000, 1)])
+def test_fft_mixed_streaming_many_rows(N, fpb):
+    """Mixed-radix lengths of 200..8192 points run in persistent workgroups
+    that load the next group of fpb rows while transforming the current one:
+    batches of 3 * 1024 groups + 1 give every workgroup several groups and a
+    partial last group.  Sampled rows against the FP64 oracle, forward out of
+    place and inverse in place, and the same input row transformed
+    bit-identically in the first group and in the partial last one (radix 8
+    first with the first pass from registers: 243 is not, 6561 = 3^8 has
+    radix 3 only)."""
+    batch = 3 * 1024 * fpb + 1
+    print(f"fft mixed stream N={N} batch={batch}", flush=True)
+    rng = np.random.default_rng(N + 11)
+    x = (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N))).astype(np.complex64)
+    x[-1] = x[0]
+    plan = mdtfx.FftPlan(N, batch)
+    plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    xi = torch.from_numpy(x).cuda()
+    X = _host(plan.exec_c2c(xi, torch.empty_like(xi), mdtfx.FFT_FORWARD))
+    rows = np.unique(np.concatenate([[0, 1, batch - 2, batch - 1], rng.integers(0, batch, 20)]))
+    bound = 2.5e-7 * np.log2(N)
+    err = _fft_err(X[rows], orc.fft_f64(x[rows].astype(np.complex128), -1))
+    print(f"  forward relative L2 error {err:.3e} (bound {bound:.3e})", flush=True)
+    assert err < bound, err
+    assert np.array_equal(X[0].view(np.uint32), X[-1].view(np.uint32))
+    Xi = _host(plan.exec_c2c(xi, xi, mdtfx.FFT_INVERSE))
+    erri = _fft_err(Xi[rows], orc.fft_f64(x[rows].astype(np.complex128), +1))
+    assert erri < bound, erri
+    assert np.array_equal(Xi[0].view(np.uint32), Xi[-1].view(np.uint32))
+    plan.destroy()
+
+
 @pytest.mark.parametrize("N", [1000, 1536, 3000, 4099])
 def test_fft_any_numpy_golden(golden, N):
     x = golden[f"fft_in_N{N}"].astype(np.complex64)
