@@ -4,6 +4,7 @@
 // natural order, forward sign -1, in place or out of place (see fft_any.hpp).
 #include <cmath>
 #include <complex>
+#include <mutex>
 #include <vector>
 
 #include "fft_any.hpp"
@@ -479,6 +480,22 @@ static int upload(const std::vector<float2> &h, float2 **d) {
 
 static float2 to_f2(std::complex<double> z) { return make_float2((float)z.real(), (float)z.imag()); }
 
+// The dynamic-LDS limit belongs to a kernel, not to a plan: it is set once,
+// to all of a gfx950 CU's LDS (160 KiB, more than any plan uses), so that a
+// plan created later never lowers it below what an earlier one needs.
+static int allow_full_lds() {
+  static std::once_flag once;
+  static bool ok = false;
+  std::call_once(once, [] {
+    const void *ks[] = {(const void *)fft_mixed_kernel<false>, (const void *)fft_mixed_kernel<true>,
+                        (const void *)fft_mixed_stream_kernel<false>, (const void *)fft_mixed_stream_kernel<true>};
+    ok = true;
+    for (const void *k : ks)
+      ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+  });
+  return ok ? MDT_SUCCESS : fail(MDT_ERR_DEVICE, "LDS limit of the mixed-radix FFT kernels");
+}
+
 // The streamed mixed-radix plan where it fits: ceil(N / 8) threads a row
 // (at most 1024), the per-pass twiddle table and FPB padded rows in LDS.
 // Otherwise, and below kStreamMinN points, the plan keeps the one-shot
@@ -509,11 +526,6 @@ static int mixed_stream_create(FftAny *p) {
   // j / NS by the multiply-high is exact while j NS < 2^32 (j < N)
   if ((unsigned long long)N * (unsigned long long)ns >= (1ULL << 32)) return MDT_SUCCESS;
   int dev = 0, cus = 0, per = 0;
-  for (int inv = 0; inv < 2; inv++) {
-    const void *k = inv ? (const void *)fft_mixed_stream_kernel<true> : (const void *)fft_mixed_stream_kernel<false>;
-    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-      return fail(MDT_ERR_DEVICE, "LDS of %zu bytes for the %d-point FFT", lds, N);
-  }
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                                                hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fft_mixed_stream_kernel<false>, tpb * fpb, lds) !=
@@ -550,11 +562,8 @@ int fft_any_create(int N, FftAny *p) {
     p->FPB = p->TPB >= 256 ? 1 : 256 / p->TPB;
     const int stride = N + (N >> 4) + 1;  // lpad(N - 1) < stride
     p->lds_bytes = sizeof(float2) * (size_t)stride * p->FPB;
-    for (int inv = 0; inv < 2; inv++) {
-      const void *k = inv ? (const void *)fft_mixed_kernel<true> : (const void *)fft_mixed_kernel<false>;
-      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes) != hipSuccess)
-        return fail(MDT_ERR_DEVICE, "LDS of %zu bytes for the %d-point FFT", p->lds_bytes, N);
-    }
+    if (p->lds_bytes > 160 * 1024) return fail(MDT_ERR_UNSUPPORTED, "%d-point FFT: LDS of %zu bytes", N, p->lds_bytes);
+    MDT_TRY(allow_full_lds());
     MDT_TRY(make_twiddles(N, &p->tw));
     return mixed_stream_create(p);
   }

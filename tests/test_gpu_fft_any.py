@@ -88,6 +88,22 @@ def test_fft_mixed_streaming_many_rows(N, fpb):
     plan.destroy()
 
 
+def test_fft_any_plans_keep_their_lds():
+    """plans with large LDS needs (8000 points streamed: 132 KiB; 16000 one-shot:
+    136 KiB) still run after smaller plans of the same kernels are created"""
+    rng = np.random.default_rng(5)
+    big = {N: mdtfx.FftPlan(N, 3) for N in (8000, 16000)}
+    small = [mdtfx.FftPlan(N, 3) for N in (1000, 10125)]
+    for N, plan in big.items():
+        x = (rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))).astype(np.complex64)
+        xi = torch.from_numpy(x).cuda()
+        X = _host(plan.exec_c2c(xi, torch.empty_like(xi)))
+        assert _fft_err(X, orc.fft_f64(x.astype(np.complex128), -1)) < 2.5e-7 * np.log2(N), N
+        plan.destroy()
+    for plan in small:
+        plan.destroy()
+
+
 @pytest.mark.parametrize("N", [1000, 1536, 3000, 4099])
 def test_fft_any_numpy_golden(golden, N):
     x = golden[f"fft_in_N{N}"].astype(np.complex64)
