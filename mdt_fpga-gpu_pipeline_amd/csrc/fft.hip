@@ -5,7 +5,7 @@
 // One workgroup holds FPB transforms of N points (N/P threads each, P points
 // per thread, see fft_core.hpp); rows are read and written straight from/to
 // global memory by the first/last Stockham pass, 64-lane contiguous.  At
-// N = 256 .. 4096 the workgroups are persistent and load their next rows
+// N = 256 .. 8192 the workgroups are persistent and load their next rows
 // while transforming the current ones (fft_stream_kernel).
 #include <atomic>
 #include <new>
@@ -46,15 +46,46 @@ __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_kernel(const float2 *i
 // operations only) and the twiddles come from the per-pass LDS table, filled
 // once per workgroup from the same W_N^k table.  The butterflies, twiddle
 // values and operation order are fft_kernel's, so the bits are too.
+// The twiddle source of fft_stream_kernel<N>: the whole per-pass table in
+// LDS where it fits beside the exchange buffers (N <= 4096); otherwise
+// (8192 points: 100 KB of table beside 68 KB of exchange buffer) the table
+// without the remainder pass, whose twiddles each thread keeps in registers
+// (RemRegTw), loaded once per workgroup.
+template <int N>
+constexpr bool stream_rem_tw() {
+  using C = FftCfg<N>;
+  return (C::FPB * lds_entries<N>() + pass_tw_entries<N, C::P>()) * 8 > 160 * 1024;
+}
+template <int N, bool REM = stream_rem_tw<N>()>
+struct StreamTw {
+  using type = PassTw<FftCfg<N>::P>;
+  static constexpr int ENTRIES = pass_tw_entries<N, FftCfg<N>::P>();
+  __device__ static type make(const float2 *tab, const float2 *, int) { return type{tab}; }
+};
+template <int N>
+struct StreamTw<N, true> {
+  using type = RemRegTw<N, FftCfg<N>::P>;
+  static constexpr int ENTRIES = type::ENTRIES;
+  __device__ static type make(const float2 *tab, const float2 *tw, int t) {
+    type r{tab, {}};
+    r.load(tw, t);
+    return r;
+  }
+};
+
 template <int N, bool INV>
 __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_stream_kernel(const float2 *in, float2 *out,
                                                                         long batch,
                                                                         const float2 *__restrict__ tw) {
   using C = FftCfg<N>;
+  using ST = StreamTw<N>;
   constexpr int P = C::P, TPB = C::TPB;
   __shared__ float2 lds[C::FPB * lds_entries<N>()];
-  __shared__ float2 twl[pass_tw_entries<N, P>()];
-  fill_pass_tw<N, P>(twl, tw, threadIdx.x, C::THREADS);
+  __shared__ float2 twl[ST::ENTRIES];
+  fill_pass_tw<N, P, ST::ENTRIES>(twl, tw, threadIdx.x, C::THREADS);
+  // (register twiddles: loaded before the barrier, whose release fence waits
+  // for them, so that no wait inside the loop counts them)
+  const auto twv = ST::make(twl, tw, threadIdx.x % TPB);
   __syncthreads();
   const int t = threadIdx.x % TPB;
   const int f = threadIdx.x / TPB;
@@ -75,7 +106,7 @@ __global__ __launch_bounds__(FftCfg<N>::THREADS) void fft_stream_kernel(const fl
     for (int m = 0; m < P; m++) xv[m] = src[m * TPB];
   };
   auto step = [&](float2 (&xv)[P], long grp) {
-    fft_block_tw<N, P, INV, true>(xv, t, lds + f * lds_entries<N>(), PassTw<P>{twl});
+    fft_block_tw<N, P, INV, true>(xv, t, lds + f * lds_entries<N>(), twv);
     long row = grp * C::FPB + f;
     row = row < batch ? row : batch - 1;
     float2 *dst = out + row * N + t;
@@ -130,7 +161,7 @@ template <int N>
 static int launch_fft(const float2 *in, float2 *out, long batch, const float2 *tw, bool inv,
                       hipStream_t st) {
   using C = FftCfg<N>;
-  if constexpr (N >= 256 && N <= 4096) {
+  if constexpr (N >= 256 && N <= 8192) {
     const long groups = (batch + C::FPB - 1) / C::FPB;
     const long grid = inv ? stream_grid<N, true>() : stream_grid<N, false>();
     if (grid > 0) {

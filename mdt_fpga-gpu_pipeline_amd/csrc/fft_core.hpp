@@ -146,10 +146,12 @@ constexpr int lds_entries() {
 // W_{NS*R}^{(j mod NS)*r}: for a given r, lanes with consecutive j read
 // consecutive words (no bank conflicts); radix-P passes hold P slots of NS
 // entries, the remainder pass R.
+// get(jm, r, q): the twiddle of input r of the thread's q-th butterfly of
+// the pass, j mod NS = jm.
 struct GlobalTw {
   const float2 *__restrict__ tw;
   template <int N, int NS, int R>
-  __device__ __forceinline__ float2 get(int jm, int r) const {
+  __device__ __forceinline__ float2 get(int jm, int r, int) const {
     return tw[jm * r * (N / (NS * R))];
   }
 };
@@ -195,7 +197,7 @@ template <int P>
 struct PassTw {
   const float2 *tab;  // LDS
   template <int N, int NS, int R>
-  __device__ __forceinline__ float2 get(int jm, int r) const {
+  __device__ __forceinline__ float2 get(int jm, int r, int) const {
     constexpr int pass = ilog2_c(NS) / ilog2_c(P);
     return tab[pass_tw_offset<P>(pass) + r * NS + jm];
   }
@@ -242,7 +244,7 @@ __device__ __forceinline__ void fft_pass(float2 (&x)[P], int t, float2 *lds, con
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        float2 w = tw.template get<N, NS, R>(jm, r);
+        float2 w = tw.template get<N, NS, R>(jm, r, q);
         if constexpr (INV) w.y = -w.y;
         v[r] = cmul(v[r], w);
       }
@@ -360,9 +362,40 @@ struct LastRegTw {
   const float2 *tab;  // LDS, passes 1 .. L-2
   float2 w[P];        // W_N^{t r}, r = 0 .. P-1
   template <int N_, int NS, int R>
-  __device__ __forceinline__ float2 get(int jm, int r) const {
+  __device__ __forceinline__ float2 get(int jm, int r, int) const {
     if constexpr (NS * P == N && R == P) {
       return w[r];
+    } else {
+      constexpr int pass = ilog2_c(NS) / ilog2_c(P);
+      return tab[pass_tw_offset<P>(pass) + r * NS + jm];
+    }
+  }
+};
+
+// Twiddles of the radix-P passes from the per-pass LDS table (as PassTw),
+// those of the remainder pass (radix RL = N / P^L, the last) from
+// registers: there NS = N/RL and thread t's butterflies j = t + q*TPB,
+// q < P/RL, use W_N^{j r}, the same every transform.  For transforms whose
+// whole table does not fit in LDS beside the exchange buffer.
+template <int N, int P>
+struct RemRegTw {
+  static constexpr int L = full_passes(N, P);
+  static constexpr int RL = N / ipow_c(P, L);
+  static_assert(RL > 1, "a remainder pass");
+  static constexpr int NW = (P / RL) * (RL - 1);
+  static constexpr int ENTRIES = pass_tw_offset<P>(L);  // LDS table: passes 1 .. L-1
+  const float2 *tab;
+  float2 w[NW];  // [q][r - 1]: W_N^{(t + q TPB) r}
+  __device__ __forceinline__ void load(const float2 *__restrict__ tw, int t) {
+#pragma unroll
+    for (int q = 0; q < P / RL; q++)
+#pragma unroll
+      for (int r = 1; r < RL; r++) w[q * (RL - 1) + r - 1] = tw[((t + q * (N / P)) & (N / RL - 1)) * r];
+  }
+  template <int N_, int NS, int R>
+  __device__ __forceinline__ float2 get(int jm, int r, int q) const {
+    if constexpr (NS * RL == N && R == RL) {
+      return w[q * (RL - 1) + r - 1];
     } else {
       constexpr int pass = ilog2_c(NS) / ilog2_c(P);
       return tab[pass_tw_offset<P>(pass) + r * NS + jm];

@@ -119,7 +119,7 @@ def test_fft_vs_fp64(N):
     plan.destroy()
 
 
-@pytest.mark.parametrize("N,fpb", [(256, 8), (512, 4), (1024, 4), (2048, 2), (4096, 1)])
+@pytest.mark.parametrize("N,fpb", [(256, 8), (512, 4), (1024, 4), (2048, 2), (4096, 1), (8192, 1)])
 def test_fft_streaming_many_rows(N, fpb):
     """Batches large enough that each persistent workgroup runs several groups
     of rows, with a partial last group (batch = 3 * 1024 groups * fpb + 1):
