@@ -2,6 +2,7 @@
 // cufftPlan1d / cufftExecC2C (MDT_OneStation_client.cu:622, 1105) accepts any
 // Nchannels in [32, 16384] (MDT_OneStation_client.h:56-57).  Unnormalised,
 // natural order, forward sign -1, in place or out of place (see fft_any.hpp).
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <mutex>
@@ -374,6 +375,89 @@ __global__ __launch_bounds__(FftCfg<M>::THREADS) void fft_bluestein_kernel(const
   }
 }
 
+// Streaming form for 512 <= M <= 4096 (N in [257, 2048]; a transform spans
+// whole waves, so the row and its buffer descriptor are wave-uniform), the
+// fft_stream_kernel pattern:
+// persistent workgroups load their next rows into registers while
+// transforming the current ones; twiddles (per-pass table), the filter and
+// the chirp sit in LDS, filled once per workgroup, so nothing inside the
+// transforms waits for the rows in flight.  Rows are read and written with
+// buffer instructions bounded by the row's N points (reads past it return
+// zero, writes past it are dropped), so every step issues the same loads
+// and stores and the compiler's waits count exactly.  Same tables, products
+// and transforms as fft_bluestein_kernel, so the same bits.
+constexpr int kBlueStreamMinM = 512, kBlueStreamMaxM = 4096;
+typedef unsigned blue_u2 __attribute__((ext_vector_type(2)));
+
+template <int M>
+__global__ __launch_bounds__(FftCfg<M>::THREADS) void fft_bluestein_stream_kernel(
+    const float2 *in, float2 *out, long batch, int N, const float2 *__restrict__ chirp,
+    const float2 *__restrict__ filt, const float2 *__restrict__ twm) {
+  using C = FftCfg<M>;
+  constexpr int P = C::P, TPB = C::TPB, H = P / 2;  // points t + m TPB < N <= M/2: m < H
+  static_assert(TPB % 64 == 0, "one row per wave at most");
+  __shared__ float2 lds[C::FPB * lds_entries<M>()];
+  __shared__ float2 twl[pass_tw_entries<M, P>()];
+  __shared__ float2 fl[M];
+  __shared__ float2 cl[M / 2];
+  fill_pass_tw<M, P>(twl, twm, threadIdx.x, C::THREADS);
+  for (int i = threadIdx.x; i < M; i += C::THREADS) fl[i] = filt[i];
+  for (int i = threadIdx.x; i < N; i += C::THREADS) cl[i] = chirp[i];
+  __syncthreads();
+  const int t = threadIdx.x % TPB;
+  const int f = __builtin_amdgcn_readfirstlane(threadIdx.x / TPB);  // wave-uniform (TPB % 64 == 0)
+  float2 *l = lds + f * lds_entries<M>();
+  const long ngroups = (batch + C::FPB - 1) / C::FPB;
+  const long G = gridDim.x;
+  auto row_of = [&](long grp) {
+    const long row = grp * C::FPB + f;
+    return row < batch ? row : batch - 1;  // past the batch: row batch - 1 again, same result
+  };
+  auto load = [&](float2 (&xv)[H], long grp) {
+    grp = grp < ngroups ? grp : grp - G;  // past the last group: the current one again (discarded)
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(in + row_of(grp) * N), (short)0, N * 8, 0x00020000);
+#pragma unroll
+    for (int m = 0; m < H; m++)
+      xv[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (t + m * TPB) * 8, 0, 0));
+  };
+  auto step = [&](const float2 (&xv)[H], long grp) {
+    float2 x[P];
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+      if (m < H) {  // (compile time) branch-free: the product, then +0 past N
+        const int n = t + m * TPB;
+        const float2 v = cmul(xv[m < H ? m : 0], cl[n < N ? n : 0]);
+        x[m] = n < N ? v : make_float2(0.f, 0.f);
+      } else {
+        x[m] = make_float2(0.f, 0.f);
+      }
+    }
+    fft_block_tw<M, P, false, true>(x, t, l, PassTw<P>{twl});
+#pragma unroll
+    for (int m = 0; m < P; m++) x[m] = cmul(x[m], fl[t + m * TPB]);
+    fft_block_tw<M, P, true, true>(x, t, l, PassTw<P>{twl});
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(out + row_of(grp) * N), (short)0, N * 8, 0x00020000);
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+      const int k = t + m * TPB;
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(blue_u2, cmul(x[m], cl[k < N ? k : 0])), r, k * 8,
+                                            0, 0);
+    }
+  };
+  float2 xa[H], xb[H];
+  long g = blockIdx.x;
+  if (g < ngroups) load(xa, g);
+  for (; g < ngroups; g += 2 * G) {  // uniform: every thread runs the same barriers
+    load(xb, g + G);
+    step(xa, g);
+    if (g + G >= ngroups) break;
+    load(xa, g + 2 * G);
+    step(xb, g + G);
+  }
+}
+
 // N > 8192: M = 32768 = 2H.  With a_n = 0 for n >= N and N <= H,
 //   A_{2k} = FFT_H(a)_k,  A_{2k+1} = FFT_H(a_n W_M^n)_k,
 //   c_k = IFFT_H(C_even)_k + W_M^{-k} IFFT_H(C_odd)_k  (k < H),
@@ -616,6 +700,28 @@ void fft_any_destroy(FftAny *p) {
 template <int M>
 static int launch_blue(const FftAny &p, const float2 *in, float2 *out, long batch, bool inv, hipStream_t st) {
   using C = FftCfg<M>;
+  if constexpr (M >= kBlueStreamMinM && M <= kBlueStreamMaxM) {
+    // resident workgroups (CUs x occupancy), queried once per device
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return fail(MDT_ERR_DEVICE, "current device");
+    int grid = cached[dev].load(std::memory_order_relaxed);
+    if (grid == 0) {
+      int cus = 0, per = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fft_bluestein_stream_kernel<M>, C::THREADS, 0) !=
+              hipSuccess ||
+          cus <= 0 || per <= 0)
+        return fail(MDT_ERR_DEVICE, "occupancy of the %d-point Bluestein transform", M);
+      grid = cus * per;
+      cached[dev].store(grid, std::memory_order_relaxed);
+    }
+    const long groups = (batch + C::FPB - 1) / C::FPB;
+    const long g = groups < grid ? groups : grid;
+    hipLaunchKernelGGL(fft_bluestein_stream_kernel<M>, dim3((unsigned)g), dim3(C::THREADS), 0, st, in, out, batch,
+                       p.N, p.chirp[inv], p.filt[inv], p.tw);
+    return check_launch("fft_bluestein_stream_kernel");
+  }
   const long blocks = (batch + C::FPB - 1) / C::FPB;
   MDT_TRY(check_grid(blocks, C::THREADS, "FFT batch"));
   hipLaunchKernelGGL(fft_bluestein_kernel<M>, dim3((unsigned)blocks), dim3(C::THREADS), 0, st, in, out, batch, p.N,

@@ -9,9 +9,10 @@
 //     global memory;
 //   * any other length: Bluestein's chirp-z algorithm, the length-N DFT as a
 //     circular convolution of length M >= 2N-1 (M a power of two) done with
-//     the power-of-two Stockham core in LDS; for N > 8192 the M = 32768
-//     convolution is split into its even and odd halves, two 16384-point
-//     transforms each way.
+//     the power-of-two Stockham core in LDS (M = 512..4096: persistent
+//     workgroups loading the next rows during the transforms); for N > 8192
+//     the M = 32768 convolution is split into its even and odd halves, two
+//     16384-point transforms each way.
 // Tables (twiddles, chirps and the transformed chirp filter) are evaluated in
 // FP64 and rounded once to FP32 at plan creation.
 #pragma once

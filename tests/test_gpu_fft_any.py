@@ -56,18 +56,21 @@ def test_fft_any_length_vs_fp64(N):
     plan.destroy()
 
 
-@pytest.mark.parametrize("N,fpb", [(243, 8), (1000, 2), (1536, 1), (3000, 1), (6561, 1), (7000, 1), (8000, 1)])
-def test_fft_mixed_streaming_many_rows(N, fpb):
-    """Mixed-radix lengths of 200..8192 points run in persistent workgroups
+@pytest.mark.parametrize("N,fpb", [(243, 8), (1000, 2), (1536, 1), (3000, 1), (6561, 1), (7000, 1), (8000, 1),
+                                   (257, 4), (509, 4), (1009, 2), (2039, 1)])
+def test_fft_any_streaming_many_rows(N, fpb):
+    """Mixed-radix lengths of 200..8192 points and Bluestein lengths of
+    257..2048 points (convolutions of 512..4096) run in persistent workgroups
     that load the next group of fpb rows while transforming the current one:
     batches of 3 * 1024 groups + 1 give every workgroup several groups and a
     partial last group.  Sampled rows against the FP64 oracle, forward out of
     place and inverse in place, and the same input row transformed
-    bit-identically in the first group and in the partial last one (radix 8
-    first with the first pass from registers: 243 is not, 6561 = 3^8 has
-    radix 3 only)."""
+    bit-identically in the first group and in the partial last one (mixed
+    radix: radix 8 first with the first pass from registers; 243 is not,
+    6561 = 3^8 has radix 3 only; Bluestein: primes 257 (M = 512), 509
+    (1024), 1009 (2048), 2039 (4096))."""
     batch = 3 * 1024 * fpb + 1
-    print(f"fft mixed stream N={N} batch={batch}", flush=True)
+    print(f"fft any stream N={N} batch={batch}", flush=True)
     rng = np.random.default_rng(N + 11)
     x = (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N))).astype(np.complex64)
     x[-1] = x[0]
