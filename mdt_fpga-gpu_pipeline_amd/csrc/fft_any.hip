@@ -609,12 +609,16 @@ static int mixed_stream_create(FftAny *p) {
   if (lds > 160 * 1024) return MDT_SUCCESS;
   // j / NS by the multiply-high is exact while j NS < 2^32 (j < N)
   if ((unsigned long long)N * (unsigned long long)ns >= (1ULL << 32)) return MDT_SUCCESS;
-  int dev = 0, cus = 0, per = 0;
+  // resident workgroups: the smaller occupancy of the two directions
+  int dev = 0, cus = 0, per = 0, per_inv = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                                                hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fft_mixed_stream_kernel<false>, tpb * fpb, lds) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_inv, fft_mixed_stream_kernel<true>, tpb * fpb, lds) !=
           hipSuccess)
     return fail(MDT_ERR_DEVICE, "occupancy of the %d-point FFT", N);
+  per = per_inv < per ? per_inv : per;
   if (cus <= 0 || per <= 0) return MDT_SUCCESS;
   if (off > 0) {
     if (hipMalloc(&p->twp, sizeof(float2) * (size_t)off) != hipSuccess)
