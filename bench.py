@@ -613,7 +613,7 @@ def fengine_shapes_leg(torch, mdtfx, reps=5):
     """The F-engine alone (int8 in, FP32 or FP16 hi/lo X input out) across
     channel and tap counts, each precision's default X-input layout, timed
     with HIP events over `reps` launches after two untimed ones.  The library
-    runs the persistent kernel at 512-4096 channels (any tap count 4-128) and
+    runs the persistent kernel at 256-4096 channels (any tap count 4-128) and
     the generic fused kernel elsewhere; "kernel" restates that rule."""
     rows = []
     for nn, n, t, ns in FE_SHAPES:
@@ -621,7 +621,7 @@ def fengine_shapes_leg(torch, mdtfx, reps=5):
         x8 = mdtfx.synth_i8(nn, n, ns + t - 1, seed=5, sigma=16.0)
         alg = nn * (ns + t - 1) * n * 2 + t * n * 4 + nn * ns * n * 8
         row = {"inputs": nn, "channels": n, "taps": t, "spectra": ns,
-               "kernel": "persistent" if 512 <= n <= 4096 else "generic", "algorithmic_bytes": alg}
+               "kernel": "persistent" if 256 <= n <= 4096 else "generic", "algorithmic_bytes": alg}
         for name, xp in (("fp32", mdtfx.XPREC_FP32), ("fp16_split", mdtfx.XPREC_FP16_SPLIT)):
             plan = mdtfx.FxPlan(nn, n, t, ns, mdtfx.WINDOW_BH4, xp)
             xin = plan.alloc_xinput()

@@ -176,9 +176,10 @@ __global__ __launch_bounds__(FCfg<N>::THREADS) void fengine_kernel(FArgs a) {
   }
 }
 
-// Persistent kernel for N in [512, 4096] (P = fft_points(N) points per thread:
-// 16 at N = 1024, so that a transform is one wave and its exchanges need no
-// workgroup barrier; 8 otherwise; TPB = N/P threads per transform) and T in
+// Persistent kernel for N in [256, 4096] (P = fft_points(N) points per thread:
+// 16 at N = 1024 and 4 at N = 256, so that a transform is one wave and its
+// exchanges need no workgroup barrier; 8 otherwise; TPB = N/P threads per
+// transform, at least one wave, so that a sub-block is wave-uniform) and T in
 // {4, 8}.  A workgroup is Q sub-blocks of TPB
 // threads; an item is one input and 4Q consecutive spectra, sub-block q
 // taking spectra 4q..4q+3.  Each sub-block runs the FIR and the four
@@ -247,7 +248,7 @@ __global__ void arrange_taps_kernel(int N, int T, const float *__restrict__ W, f
   Wt[idx] = i < T ? W[(long)i * N + t + m * TPB] : 0.0f;
 }
 
-bool persistent_shape(int N, int T) { return N >= 512 && N <= 4096 && T >= 4 && T <= MDT_MAX_TAPS; }
+bool persistent_shape(int N, int T) { return N >= 256 && N <= 4096 && T >= 4 && T <= MDT_MAX_TAPS; }
 int arranged_taps_floats(int N, int T) { return N * 4 * ((T + 3) / 4); }
 bool fp8_block_shape(int N) { return N == 512 || N == 1024; }
 
@@ -495,7 +496,7 @@ fengine_persistent_kernel(FArgs a, long nitems) {
         (void *)(a.in + (long)s0_ * spec + (long)e_ * N), (short)0, 0x7fffffff, kBufferWord3);
     const int tid3 = opaque(tid);
     const int wv = __builtin_amdgcn_readfirstlane(tid3 >> 6), lane = tid3 & 63;
-    constexpr int KPF = N * 2 / 1024;  // wave-instructions per frame
+    constexpr int KPF = N * 2 >= 1024 ? N * 2 / 1024 : 1;  // wave-instructions per frame (DMA: N >= 1024)
     for (int c = wv; c < NF * KPF; c += C::THREADS / 64) {
       const int f = c / KPF, k = c % KPF;
       const int fr = f < fmax_ ? f : fmax_;
@@ -967,7 +968,7 @@ static int launch_persistent(const FArgs &a, hipStream_t st) {
       if constexpr (N >= 2048) return launch_persistent_s<N, T, 2>(a, st);
       break;
     case 3:
-      if constexpr (N <= 1024) return launch_persistent_s<N, T, 3>(a, st);
+      if constexpr (N >= 512 && N <= 1024) return launch_persistent_s<N, T, 3>(a, st);
       break;
     default: break;
   }
@@ -978,7 +979,7 @@ template <int N>
 static int launch_fengine_n(const FArgs &a, hipStream_t st) {
   using C = FCfg<N>;
   const bool fin = a.fin != nullptr;
-  if constexpr (N >= 512 && N <= 4096) {
+  if constexpr (N >= 256 && N <= 4096) {
     if (!fin && a.T == 4) return launch_persistent<N, 4>(a, st);
     if (!fin && a.T == 8) return launch_persistent<N, 8>(a, st);
     if (!fin && persistent_shape(N, a.T)) return launch_persistent<N, 0>(a, st);

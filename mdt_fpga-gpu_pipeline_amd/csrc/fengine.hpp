@@ -72,7 +72,7 @@ __device__ __forceinline__ float2 split_sample(float2 v, float sc) {
 
 // int8 input (a.in) or float2 input (a.fin, the generic kernel only)
 int launch_fengine(const FArgs &a, int N, hipStream_t st);
-// shapes served by the persistent kernel (N in [512, 4096], any tap count;
+// shapes served by the persistent kernel (N in [256, 4096], any tap count;
 // 4 and 8 taps with their own code, the others in groups of four)
 bool persistent_shape(int N, int T);
 // floats of the arranged window (taps zero-padded to a multiple of 4)

@@ -32,9 +32,13 @@ constexpr int full_passes(int n, int p) {
 // points per thread for an N-point transform: 16 at N = 1024, so that one
 // transform is one wave (64 threads) and its LDS exchanges need no workgroup
 // barrier, and at N = 2048 (three passes, two exchanges instead of three);
-// 8 up to 4096 (a 512-point transform is one wave already; at 4096 the
-// F-engine's FP16 hi/lo output measured twice as slow with 16); 16 above
-constexpr int fft_points(int N) { return (N == 1024 || N == 2048) ? 16 : (N <= 4096 ? 8 : 16); }
+// 4 at N = 256, for the same reason (a transform is one whole wave, so the
+// persistent F-engine's sub-blocks stay wave-uniform); 8 otherwise up to
+// 4096 (a 512-point transform is one wave already; at 4096 the F-engine's
+// FP16 hi/lo output measured twice as slow with 16); 16 above
+constexpr int fft_points(int N) {
+  return (N == 1024 || N == 2048) ? 16 : (N == 256 ? 4 : (N <= 4096 ? 8 : 16));
+}
 
 // exchange synchronisation between passes: a transform held by one wave (or
 // part of one) only orders its own LDS accesses (a wave's LDS operations

@@ -148,7 +148,7 @@ static int check_xprec(int xprec, int N, int T) {
   if (xprec != MDT_XPREC_FP32 && xprec != MDT_XPREC_FP16_SPLIT && xprec != MDT_XPREC_FP8)
     return fail(MDT_ERR_INVALID_VALUE, "unknown X-engine precision %d", xprec);
   // the FP8 sample format and channel powers come from the persistent F-engine
-  if (xprec == MDT_XPREC_FP8 && !persistent_shape(N, T))
+  if (xprec == MDT_XPREC_FP8 && (N < 512 || !persistent_shape(N, T)))
     return fail(MDT_ERR_UNSUPPORTED, "FP8 path needs Nchannels in [512, 4096]");
   return MDT_SUCCESS;
 }

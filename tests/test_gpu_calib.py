@@ -129,8 +129,9 @@ def test_loss_zero_at_truth(c5):
     assert float(_host(loss)) == 0.0
 
 
-@pytest.mark.parametrize("N_,T_", [(1024, 4), (4096, 8)])
-@pytest.mark.parametrize("xprec", [mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT, mdtfx.XPREC_FP8])
+@pytest.mark.parametrize("xprec,N_,T_", [(x, n, t) for n, t in ((1024, 4), (4096, 8))
+                                         for x in (mdtfx.XPREC_FP32, mdtfx.XPREC_FP16_SPLIT, mdtfx.XPREC_FP8)]
+                         + [(mdtfx.XPREC_FP32, 256, 4), (mdtfx.XPREC_FP16_SPLIT, 256, 8)])
 def test_gain_stage_in_every_xinput_layout(xprec, N_, T_):
     """the F-engine's gain/delay stage writes the same samples in every X
     input layout (its own compiled variant per layout), and the X-engine

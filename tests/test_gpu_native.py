@@ -27,7 +27,7 @@ def _tri_autonorm_err(vis, v64, NN):
 
 @pytest.mark.parametrize("NN,N,T,Ns", [(40, 1024, 4, 256), (3, 32, 4, 11), (5, 64, 8, 17),
                                        (4, 2048, 4, 9), (3, 4096, 8, 16), (37, 4096, 8, 69), (3, 8192, 4, 5),
-                                       (3, 16384, 4, 3), (6, 256, 5, 12),
+                                       (3, 16384, 4, 3), (6, 256, 5, 12), (40, 256, 4, 100), (70, 256, 8, 200),
                                        (6, 1024, 16, 40), (5, 2048, 5, 33), (3, 4096, 12, 20), (300, 512, 7, 70),
                                        (4, 1024, 128, 9)])
 @pytest.mark.parametrize("layout", [mdtfx.XIN_ROWS, mdtfx.XIN_PAIRS, mdtfx.XIN_BLOCKS])
