@@ -113,7 +113,7 @@ int mdt_pps_batch_i8(int Nchannels, int Ntaps, const float *Window, const int8_t
  * (MDT_OneStation_client.cu:622, 630, 1105).  Batched, unnormalised, natural
  * bin order; in place or out of place.  nx: every length the client accepts,
  * [32, 16384] (MDT_OneStation_client.h:56-57), and powers of two from 8:
- * powers of two in radix-8 Stockham passes, other 7-smooth lengths
+ * powers of two in radix-16/8/4 Stockham passes, other 7-smooth lengths
  * (2^a 3^b 5^c 7^d) in mixed-radix passes, the rest by Bluestein's chirp-z
  * convolution (MDT_ERR_UNSUPPORTED outside these). */
 typedef struct mdt_fft_plan_s *mdt_fft_plan;
