@@ -585,9 +585,10 @@ static int allow_full_lds() {
 // Otherwise, and below kStreamMinN points, the plan keeps the one-shot
 // kernel.  Measured on 1.6e8-sample batches (HIP events): streamed 0.56-0.78
 // ms against 1.20-1.60 ms one-shot at 1000-8000 points, 0.77 against 1.35
-// ms at 243; at 35-125 points the one-shot kernel was as fast or faster
-// (its many small workgroups, 1.0-2.2 ms either way).
-constexpr int kStreamMinN = 200;
+// ms at 243, 0.53 against 0.82 at 192, 0.93-1.10 against 1.04-1.30 at 144,
+// 150, 180, 189 (168 and 175 within 5 % either way); at 35-140 points the
+// one-shot kernel was as fast or faster (its many small workgroups).
+constexpr int kStreamMinN = 144;
 
 static int mixed_stream_create(FftAny *p) {
   const int N = p->N;

@@ -56,10 +56,10 @@ def test_fft_any_length_vs_fp64(N):
     plan.destroy()
 
 
-@pytest.mark.parametrize("N,fpb", [(243, 8), (1000, 2), (1536, 1), (3000, 1), (6561, 1), (7000, 1), (8000, 1),
-                                   (257, 4), (509, 4), (1009, 2), (2039, 1)])
+@pytest.mark.parametrize("N,fpb", [(144, 14), (189, 10), (243, 8), (1000, 2), (1536, 1), (3000, 1), (6561, 1),
+                                   (7000, 1), (8000, 1), (257, 4), (509, 4), (1009, 2), (2039, 1)])
 def test_fft_any_streaming_many_rows(N, fpb):
-    """Mixed-radix lengths of 200..8192 points and Bluestein lengths of
+    """Mixed-radix lengths of 144..8192 points and Bluestein lengths of
     257..2048 points (convolutions of 512..4096) run in persistent workgroups
     that load the next group of fpb rows while transforming the current one:
     batches of 3 * 1024 groups + 1 give every workgroup several groups and a
