@@ -178,6 +178,128 @@ __global__ __launch_bounds__(1024) void fft_mixed_kernel(MixedArgs a) {
   }
 }
 
+// ------------------------------------------------------------ mixed, short
+// Below kStreamMinN points the one-shot form again, compiled per length: the
+// radix sequence, stage spans, butterfly counts and twiddle strides are
+// constants, the passes inlined, so j mod NS and the table indices cost no
+// run-time division and no call.  Same passes, twiddles and order as
+// fft_mixed_kernel (same bits).
+struct MixedSeq {
+  int n, rad[kMaxPasses];
+};
+constexpr MixedSeq mixed_seq(int N) {  // fft_factor's sequence: 8s, a 4 or 2, 3s, 5s, 7s
+  MixedSeq s{0, {}};
+  int n = N;
+  while (n % 8 == 0) s.rad[s.n++] = 8, n /= 8;
+  if (n % 4 == 0) s.rad[s.n++] = 4, n /= 4;
+  if (n % 2 == 0) s.rad[s.n++] = 2, n /= 2;
+  for (int r = 3; r <= 7; r += 2)
+    while (n % r == 0) s.rad[s.n++] = r, n /= r;
+  return s;
+}
+template <int N, int PASS>
+constexpr int seq_ns() {
+  int ns = 1;
+  for (int p = 0; p < PASS; p++) ns *= mixed_seq(N).rad[p];
+  return ns;
+}
+
+template <int N, bool INV, int PASS>
+__device__ __forceinline__ void ms_short_pass(const float2 *src, float2 *dst, float2 *lds, bool valid, int t,
+                                              const float2 *__restrict__ tw) {
+  constexpr MixedSeq S = mixed_seq(N);
+  constexpr int R = S.rad[PASS], NS = seq_ns<N, PASS>(), TPB = (N + 7) / 8, nb = N / R;
+  constexpr int QM = (nb + TPB - 1) / TPB;
+  constexpr bool FROM_LDS = PASS > 0, TO_LDS = PASS + 1 < S.n;
+  float2 v[QM][R];
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int j = t + q * TPB;
+    if (j < nb) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = j + r * nb;
+        v[q][r] = FROM_LDS ? lds[lpad(i)] : (valid ? src[i] : make_float2(0.f, 0.f));
+      }
+    }
+  }
+  if (FROM_LDS && TO_LDS) __syncthreads();
+#pragma unroll
+  for (int q = 0; q < QM; q++) {
+    const int j = t + q * TPB;
+    if (j < nb) {
+      const int jm = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          float2 w = tw[jm * r * (N / (NS * R))];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      dft_any<R, INV>(v[q]);
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int o = base + r * NS;
+        if (TO_LDS)
+          lds[lpad(o)] = v[q][r];
+        else if (valid)
+          store_stream(dst + o, v[q][r]);
+      }
+    }
+  }
+  if (TO_LDS) __syncthreads();
+  if constexpr (PASS + 1 < S.n) ms_short_pass<N, INV, PASS + 1>(src, dst, lds, valid, t, tw);
+}
+
+template <int N, bool INV>
+__global__ __launch_bounds__(256) void fft_mixed_short_kernel(const float2 *in, float2 *out, long batch,
+                                                              const float2 *__restrict__ tw) {
+  constexpr int TPB = (N + 7) / 8, FPB = 256 / TPB, STRIDE = N + (N >> 4) + 1;
+  static_assert(mixed_seq(N).n >= 2, "at least two passes");
+  __shared__ float2 lds[FPB * STRIDE];
+  const int f = threadIdx.x / TPB, t = threadIdx.x % TPB;
+  const long row = (long)blockIdx.x * FPB + f;
+  const bool valid = row < batch;
+  ms_short_pass<N, INV, 0>(in + (valid ? row : 0) * N, out + (valid ? row : 0) * N, lds + f * STRIDE, valid, t,
+                           tw);
+}
+
+template <int N>
+static int launch_mixed_short(const float2 *tw, const float2 *in, float2 *out, long batch, bool inv,
+                              hipStream_t st) {
+  constexpr int TPB = (N + 7) / 8, FPB = 256 / TPB;
+  const long blocks = (batch + FPB - 1) / FPB;
+  MDT_TRY(check_grid(blocks, TPB * FPB, "FFT batch"));
+  if (inv)
+    hipLaunchKernelGGL((fft_mixed_short_kernel<N, true>), dim3((unsigned)blocks), dim3(TPB * FPB), 0, st, in, out,
+                       batch, tw);
+  else
+    hipLaunchKernelGGL((fft_mixed_short_kernel<N, false>), dim3((unsigned)blocks), dim3(TPB * FPB), 0, st, in, out,
+                       batch, tw);
+  return check_launch("fft_mixed_short_kernel");
+}
+
+// the compiled short lengths: the 7-smooth lengths in [32, 143] that are not
+// powers of two and where the compiled form measured faster (1.6e8-sample
+// batches: 1.04-1.89x, e.g. 35 points 1.43 -> 0.76 ms, 63 2.11 -> 1.12, 135
+// 1.27 -> 0.88); at 72, 80, 100 and 112 points the run-time kernel was
+// faster (0.81-0.96x) and is kept.  -1 when N is not one of them.
+static int mixed_short_dispatch(int N, const float2 *tw, const float2 *in, float2 *out, long batch, bool inv,
+                                hipStream_t st) {
+  switch (N) {
+#define MDT_SHORT(n) \
+  case n: return launch_mixed_short<n>(tw, in, out, batch, inv, st);
+    MDT_SHORT(35) MDT_SHORT(36) MDT_SHORT(40) MDT_SHORT(42) MDT_SHORT(45) MDT_SHORT(48) MDT_SHORT(49)
+    MDT_SHORT(50) MDT_SHORT(54) MDT_SHORT(56) MDT_SHORT(60) MDT_SHORT(63) MDT_SHORT(70) MDT_SHORT(75)
+    MDT_SHORT(81) MDT_SHORT(84) MDT_SHORT(90) MDT_SHORT(96) MDT_SHORT(98) MDT_SHORT(105) MDT_SHORT(108)
+    MDT_SHORT(120) MDT_SHORT(125) MDT_SHORT(126) MDT_SHORT(135) MDT_SHORT(140)
+#undef MDT_SHORT
+    default: return -1;
+  }
+}
+
 // ------------------------------------------------------------ mixed, streamed
 // The same passes in persistent workgroups (N <= 8192 where the tables fit in
 // LDS): each workgroup loops over groups of FPB rows, and the next group's
@@ -758,6 +880,8 @@ int fft_any_exec(const FftAny &p, const float2 *in, float2 *out, long batch, boo
     return check_launch("fft_mixed_stream_kernel");
   }
   if (p.kind == FFT_ANY_MIXED) {
+    const int sh = mixed_short_dispatch(p.N, p.tw, in, out, batch, inv, st);
+    if (sh >= 0) return sh;
     MixedArgs a{};
     a.in = in;
     a.out = out;

@@ -28,12 +28,13 @@ def _fft_err(X, ref):
     return float(np.max(np.linalg.norm(X - ref, axis=-1) / np.linalg.norm(ref, axis=-1)))
 
 
-# mixed radix: 35 = 5*7, 48, 96, 100, 125 = 5^3, 243 = 3^5, 343 = 7^3, 1000,
+# mixed radix: 35 = 5*7, 48, 63, 96, 100, 112, 125 = 5^3, 135 (35-135: the
+# compiled short kernels, except 100 and 112), 243 = 3^5, 343 = 7^3, 1000,
 # 1536, 3000, 6561 = 3^8, 9000, 11025 = 3^2 5^2 7^2, 16000;  Bluestein: 33 =
 # 3*11 (M = 128), 4099 (prime, M = 16384), 8191 (prime, M = 16384), 12289
 # (prime, split), 16383 = 3*43*127 (split)
-LENGTHS = [33, 35, 48, 96, 100, 125, 243, 343, 1000, 1536, 3000, 4099, 6561, 8191, 9000, 11025, 12289, 16000,
-           16383]
+LENGTHS = [33, 35, 48, 63, 96, 100, 112, 125, 135, 243, 343, 1000, 1536, 3000, 4099, 6561, 8191, 9000, 11025,
+           12289, 16000, 16383]
 
 
 @pytest.mark.parametrize("N", LENGTHS)
